@@ -1,0 +1,233 @@
+"""SART throughput on one MI355X: GUPS on the 512^3 x 121-tilt workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxel-type float|half|bfloat16]
+                    [--workload metric|c2] [--generic] [--no-cpu-baseline]
+
+A step is one full iteration of the block-iterative loop over the synthetic
+tilt series (for SART: every projection's forward projection, residual and
+back projection).  GUPS = voxels x projections x steps / seconds / 1e9, with
+the volume, the measured projections and every buffer resident in HBM when
+the timed region starts.  Prints ONE JSON line on stdout; progress on stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, GB/s
+
+WORKLOADS = {
+    # SART on a 512^3 volume, 121 tilts at +-60 deg in 1 deg steps, 512^2 detector
+    "metric": dict(res=(512, 512, 512), det=(512, 512), max_angle=60.0, step=1.0, algorithm="sart",
+                   block_size=0, order="random", lam=0.3, beads=0,
+                   name="SART 512^3 volume, 121 tilts +-60 deg by 1 deg, 512^2 detector, random order"),
+    # block-iterative B=8, 512x512x256 with 200 beads
+    "c2": dict(res=(512, 512, 256), det=(512, 512), max_angle=60.0, step=1.0, algorithm="blockiterative",
+               block_size=8, order="random", lam=0.3, beads=200,
+               name="block-iterative B=8 on 512x512x256 + 200 beads, 121 tilts +-60 deg by 1 deg, 512^2 detector"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--voxel-type", default="float", choices=["float", "half", "bfloat16"])
+    ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
+    ap.add_argument("--generic", action="store_true", help="force the generic (unfused) kernels")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-projections", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (traffic calibration for rocprof counters)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the volume shards along y with "
+              "no exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ettention_amd as E
+    from ettention_amd import api as A
+    from ettention_amd import synthetic as S
+
+    wl = WORKLOADS[args.workload]
+    nx, ny, nz = wl["res"]
+    nu, nv = wl["det"]
+    V = nx * ny * nz
+    P = nu * nv
+    tilts = S.tilt_series(wl["max_angle"], wl["step"])
+    N = len(tilts)
+    vg = E.VolumeGeometry.make((nx, ny, nz))
+    base = E.ScannerGeometry.mrc_base(nu, nv)
+    dev = torch.device("cuda:0")
+
+    t0 = time.time()
+    phantom = S.ellipsoid_phantom((nz, ny, nx), seed=1000 + (1 if args.workload == "metric" else 2),
+                                  beads=wl["beads"], device=dev)
+    projs = torch.zeros((N, nv, nu), dtype=torch.float32, device=dev)
+    scratch = torch.zeros((nv, nu), dtype=torch.float32, device=dev)
+    for i, t in enumerate(tilts):
+        scratch.zero_()
+        A.forward_project(vg, phantom, A.rotate(base, float(t)), projs[i], scratch)
+    projs = S.add_gaussian_noise(projs, 0.02, seed=7)
+    torch.cuda.synchronize()
+    log("synthetic input ready in %.1f s (phantom max %.2f)" % (time.time() - t0, float(phantom.max())))
+    del phantom, scratch
+
+    flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
+    r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs,
+                                               algorithm=wl["algorithm"], block_size=wl["block_size"],
+                                               lam=wl["lam"], projection_iteration=wl["order"],
+                                               voxel_type=args.voxel_type, flags=flags)
+    fused = r.uses_fused_path()
+    for _ in range(args.warmup):
+        r.run_single_iteration()
+    r.synchronize()
+    log("warmup done (fused path: %s)" % fused)
+
+    # timed region: K iterations, synchronised on both sides
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    r.run_iterations(args.steps)
+    r.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    gups = V * N * args.steps / elapsed / 1e9
+    log("timed %d iterations: %.2f ms/iteration, %.1f GUPS" % (args.steps, ms_per_step, gups))
+    norm = r.residual_norm()
+    log("residual norm after %d iterations: %.6e" % (args.warmup + args.steps, norm))
+
+    roofline = kernel_roofline(r, vg, base, tilts, projs, args.voxel_type, fused, V, P)
+    del r
+    torch.cuda.empty_cache()
+
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB write")
+        del src, dst
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, tilts, projs, args.cpu_sample_projections)
+
+    out = {
+        "metric": "SART GUPS (proj+backproj) 512^3 x 121 tilts",
+        "value": round(gups, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"float": "f32", "half": "f16-storage/f32-accumulate",
+                  "bfloat16": "bf16-storage/f32-accumulate"}[args.voxel_type],
+        "data": "synthetic (seeded ellipsoid phantom, FP projections + 2% Gaussian noise)",
+        "config": {"workload": wl["name"], "volume": [nx, ny, nz], "detector": [nu, nv], "projections": N,
+                   "algorithm": wl["algorithm"], "block_size": wl["block_size"] or (1 if wl["algorithm"] == "sart" else N),
+                   "voxel_type": args.voxel_type, "kernels": "fused-plane" if fused else "generic",
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def kernel_roofline(r, vg, base, tilts, projs, voxel_type, fused, V, P):
+    """Average duration of the dominant kernel, measured with HIP events on the
+    stream the kernel runs on, and its algorithmic bytes per launch."""
+    import torch
+    from ettention_amd import api as A
+    s = 4 if voxel_type == "float" else 2
+    stream = torch.cuda.Stream()
+    if fused:
+        from ettention_amd import fused_probe
+        dur_ms, launches = fused_probe.time_fused_launch(r, stream)
+        name = "fused plane kernel (BP of projection k-1 + FP/residual of projection k)"
+        alg_bytes = 12 * V if s == 4 else 6 * V        # per-update figure x V updates
+        alg_bytes = (3 * s * V + 20 * P)
+        kernel_bytes = 2 * s * V + 28 * P
+    else:
+        # time the generic FP and BP kernels for a spread of tilts on a copy of the volume
+        vol = torch.zeros((vg.resolution[2], vg.resolution[1], vg.resolution[0]),
+                          dtype={"float": torch.float32, "half": torch.float16, "bfloat16": torch.bfloat16}[voxel_type],
+                          device="cuda")
+        nv, nu = projs.shape[1], projs.shape[2]
+        p = torch.zeros((nv, nu), dtype=torch.float32, device="cuda")
+        l = torch.zeros_like(p)
+        res = torch.zeros_like(p)
+        sel = list(range(0, len(tilts), max(1, len(tilts) // 16)))
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        with torch.cuda.stream(stream):
+            for k in range(2):   # first pass warms up
+                ev[0].record(stream)
+                for i in sel:
+                    A.forward_project(vg, vol, A.rotate(base, float(tilts[i])), p, l, stream=stream.cuda_stream)
+                ev[1].record(stream)
+                for i in sel:
+                    A.back_project(vg, vol, A.rotate(base, float(tilts[i])), res, l, stream=stream.cuda_stream)
+                ev[2].record(stream)
+        stream.synchronize()
+        fp_ms = ev[0].elapsed_time(ev[1]) / len(sel)
+        bp_ms = ev[1].elapsed_time(ev[2]) / len(sel)
+        log("generic kernels: FP %.3f ms, BP %.3f ms per launch" % (fp_ms, bp_ms))
+        if fp_ms >= bp_ms:
+            dur_ms, name, alg_bytes = fp_ms, "forward_project_kernel (generic DDA)", s * V + 8 * P
+        else:
+            dur_ms, name, alg_bytes = bp_ms, "back_project_kernel (generic)", 2 * s * V + 8 * P
+        kernel_bytes = alg_bytes
+        launches = len(sel)
+    achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_ms": round(dur_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes),
+            "compulsory_kernel_bytes_per_launch": int(kernel_bytes), "timed_launches": launches}
+
+
+def cpu_baseline(wl, tilts, projs, nsample):
+    """The oracle (FP32-faithful C restatement of the reference's OpenCL path,
+    OpenMP over the host cores) on a bounded sample of the same workload: the
+    first `nsample` projection steps (FP + residual + BP) of a SART pass over
+    the full-size volume."""
+    from oracle import oracle as O
+    nx, ny, nz = wl["res"]
+    nu, nv = wl["det"]
+    v = O.Volume.make((nx, ny, nz))
+    base = O.mrc_base_scanner(nu, nv)
+    vol = np.zeros((nz, ny, nx), np.float32)
+    meas = projs[:nsample].cpu().numpy()
+    t0 = time.perf_counter()
+    for i in range(nsample):
+        g = O.rotate(base, float(tilts[i]))
+        p, l = O.forward_project(v, vol, g)
+        res = O.compare(meas[i], p)
+        O.back_project(v, vol, g, res, l, lam=wl["lam"])
+    dt = time.perf_counter() - t0
+    gups = nx * ny * nz * nsample / dt / 1e9
+    log("cpu baseline: %d projection steps in %.1f s on %d threads -> %.4f GUPS" % (nsample, dt, O.num_threads(), gups))
+    return {"value": round(gups, 5), "unit": "GUPS", "cores": O.num_threads(), "kind": "port",
+            "sample": "%d SART projection steps (FP+residual+BP) on the full %dx%dx%d volume, %d^2 detector"
+                      % (nsample, nx, ny, nz, nu)}
+
+
+if __name__ == "__main__":
+    main()

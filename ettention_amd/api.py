@@ -1,0 +1,257 @@
+"""Python front end of libetn.so for tests and benchmarks.
+
+Device memory and streams come from PyTorch-ROCm tensors (plumbing only); all
+computation happens in libetn.so.  Names follow the reference operators:
+forward_project ~ ForwardProjectionOperator::Execute, back_project ~
+BackProjectionOperator::execute, BlockIterativeReconstructionOperator ~ the
+C++ class of the same name.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (ETN_ALL_SUBVOLUMES, ETN_VOXEL_BFLOAT16, ETN_VOXEL_FLOAT16, ETN_VOXEL_FLOAT32,
+                   ReconstructionDesc, ScannerGeometry, VolumeGeometry, call, lib)
+
+_VOXEL_TYPES = {"float": ETN_VOXEL_FLOAT32, "float32": ETN_VOXEL_FLOAT32, "half": ETN_VOXEL_FLOAT16,
+                "float16": ETN_VOXEL_FLOAT16, "bfloat16": ETN_VOXEL_BFLOAT16, "bf16": ETN_VOXEL_BFLOAT16}
+
+
+def voxel_type_id(vt):
+    if isinstance(vt, int):
+        return vt
+    return _VOXEL_TYPES[vt]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream=None):
+    if stream is not None:
+        return C.c_void_p(stream)
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _voxel_type_of(t):
+    torch = _torch()
+    return {torch.float32: ETN_VOXEL_FLOAT32, torch.float16: ETN_VOXEL_FLOAT16,
+            torch.bfloat16: ETN_VOXEL_BFLOAT16}[t.dtype]
+
+
+# ---- host geometry ----------------------------------------------------------
+
+def rotate(base: ScannerGeometry, tilt_deg, x_axis_tilt_deg=0.0) -> ScannerGeometry:
+    out = ScannerGeometry()
+    call("etn_rotate_scanner_geometry", C.byref(base), float(tilt_deg), float(x_axis_tilt_deg), C.byref(out))
+    return out
+
+
+def ray_direction(g: ScannerGeometry):
+    out = np.zeros(3, np.float32)
+    call("etn_ray_direction", C.byref(g), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def projection_matrix(g: ScannerGeometry):
+    out = np.zeros(16, np.float32)
+    call("etn_projection_matrix", C.byref(g), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def traversal_length_through_bbox(g: ScannerGeometry, bmin, bmax):
+    a = np.asarray(bmin, np.float32)
+    b = np.asarray(bmax, np.float32)
+    out = np.zeros(1, np.float32)
+    call("etn_traversal_length_through_bbox", C.byref(g), a.ctypes.data_as(C.c_void_p),
+         b.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+    return float(out[0])
+
+
+def bp_sample_pattern(g: ScannerGeometry, voxel_size, samples):
+    vs = np.asarray(voxel_size, np.float32)
+    out = np.zeros(4 * max(int(samples), 1), np.float32)
+    call("etn_bp_sample_pattern", C.byref(g), vs.ctypes.data_as(C.c_void_p), int(samples),
+         out.ctypes.data_as(C.c_void_p))
+    return out.reshape(-1, 4)
+
+
+def subvolume(v: VolumeGeometry, index):
+    res = np.zeros(3, np.uint32)
+    first = np.zeros(1, np.uint32)
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    call("etn_subvolume", C.byref(v), int(index), res.ctypes.data_as(C.c_void_p), first.ctypes.data_as(C.c_void_p),
+         bmin.ctypes.data_as(C.c_void_p), bmax.ctypes.data_as(C.c_void_p))
+    return res, int(first[0]), bmin, bmax
+
+
+def projection_order(scheme, count, skip=()):
+    out = np.zeros(max(int(count), 1), np.uint32)
+    n = np.zeros(1, np.uint32)
+    sk = np.asarray(list(skip) or [0], np.uint32)
+    call("etn_projection_order", scheme.encode(), int(count), sk.ctypes.data_as(C.c_void_p), len(skip),
+         out.ctypes.data_as(C.c_void_p), n.ctypes.data_as(C.c_void_p))
+    return out[: int(n[0])].copy()
+
+
+# ---- kernels ----------------------------------------------------------------
+
+def forward_project(vol_geom: VolumeGeometry, volume, g: ScannerGeometry, projection, traversal_length,
+                    samples=1, subvolume=0, mask=None, stream=None):
+    """Adds the subvolume's line integrals into projection / traversal_length
+    (float32 device tensors of shape (nv, nu))."""
+    call("etn_forward_project", C.byref(vol_geom), _voxel_type_of(volume), _ptr(volume), _ptr(mask),
+         int(subvolume), C.byref(g), int(samples), _ptr(projection), _ptr(traversal_length), _stream(stream))
+
+
+def back_project(vol_geom: VolumeGeometry, volume, g: ScannerGeometry, residual, ray_length, lam=1.0,
+                 block_size=1, samples=1, subvolume=ETN_ALL_SUBVOLUMES, mask=None, stream=None):
+    call("etn_back_project", C.byref(vol_geom), _voxel_type_of(volume), _ptr(volume), _ptr(mask),
+         int(subvolume), C.byref(g), int(samples), _ptr(residual), _ptr(ray_length), float(lam),
+         int(block_size), _stream(stream))
+
+
+def compare(real, virtual, residual, stream=None):
+    nv, nu = real.shape[-2], real.shape[-1]
+    call("etn_compare", int(nu), int(nv), _ptr(real), _ptr(virtual), _ptr(residual), _stream(stream))
+
+
+def long_object_compensate(projection, traversal_length, constant_length, stream=None):
+    nv, nu = projection.shape[-2], projection.shape[-1]
+    call("etn_long_object_compensate", int(nu), int(nv), _ptr(projection), _ptr(traversal_length),
+         float(constant_length), _stream(stream))
+
+
+def compute_ray_length(vol_geom: VolumeGeometry, g: ScannerGeometry, out, stream=None):
+    call("etn_compute_ray_length", C.byref(vol_geom), C.byref(g), _ptr(out), _stream(stream))
+
+
+def sum_squared_difference(a, b, out, stream=None):
+    call("etn_sum_squared_difference_f64", _ptr(a), _ptr(b), a.numel(), _ptr(out), _stream(stream))
+
+
+def convert_volume(src, dst, stream=None):
+    call("etn_convert_volume", _ptr(src), _voxel_type_of(src), _ptr(dst), _voxel_type_of(dst), src.numel(),
+         _stream(stream))
+
+
+# ---- reconstruction ---------------------------------------------------------
+
+class BlockIterativeReconstructionOperator:
+    """Wraps etn_reconstruction_* (BlockIterativeReconstructionOperator).
+
+    projections: float32 device tensor (count, nv, nu) in stack order;
+    tilt_angles: host sequence of degrees."""
+
+    def __init__(self, *, volume: VolumeGeometry, base_geometry: ScannerGeometry, tilt_angles, projections,
+                 algorithm="sart", block_size=0, iterations=1, lam=0.3, use_long_object_compensation=False,
+                 fp_samples=1, bp_samples=1, projection_iteration="random", skip_projections=(),
+                 voxel_type="float", init_value=0.0, x_axis_tilt=0.0, mask=None, flags=0, stream=None):
+        self._tilts = np.ascontiguousarray(tilt_angles, np.float32)
+        self._skip = np.asarray(list(skip_projections) or [0], np.uint32)
+        self._algorithm = algorithm.encode()
+        self._order = projection_iteration.encode() if projection_iteration else None
+        self._projections = projections
+        self._mask = mask
+        d = ReconstructionDesc()
+        d.algorithm = self._algorithm
+        d.block_size = int(block_size)
+        d.iterations = int(iterations)
+        d.lambda_ = float(lam)
+        d.use_long_object_compensation = int(bool(use_long_object_compensation))
+        d.fp_samples = int(fp_samples)
+        d.bp_samples = int(bp_samples)
+        d.projection_iteration = self._order
+        d.skip_projections = self._skip.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.skip_count = len(skip_projections)
+        d.voxel_type = voxel_type_id(voxel_type)
+        d.volume = volume
+        d.init_value = float(init_value)
+        d.base_geometry = base_geometry
+        d.projection_count = len(self._tilts)
+        d.tilt_angles = self._tilts.ctypes.data_as(C.POINTER(C.c_float))
+        d.x_axis_tilt = float(x_axis_tilt)
+        d.projections = _ptr(projections)
+        d.mask = _ptr(mask)
+        d.flags = int(flags)
+        self.desc = d
+        self.voxel_type = d.voxel_type
+        self.shape = (volume.resolution[2], volume.resolution[1], volume.resolution[0])
+        h = C.c_void_p()
+        call("etn_reconstruction_create", C.byref(d), C.c_void_p(stream) if stream else None, C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().etn_reconstruction_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self):
+        call("etn_reconstruction_run", self._h)
+
+    def run_single_iteration(self):
+        call("etn_reconstruction_run_single_iteration", self._h)
+
+    def run_iterations(self, n):
+        call("etn_reconstruction_run_iterations", self._h, int(n))
+
+    def synchronize(self):
+        call("etn_reconstruction_synchronize", self._h)
+
+    def residual_norm(self):
+        out = C.c_double()
+        call("etn_reconstruction_residual_norm", self._h, C.byref(out))
+        return out.value
+
+    def uses_fused_path(self):
+        out = C.c_int()
+        call("etn_reconstruction_uses_fused_path", self._h, C.byref(out))
+        return bool(out.value)
+
+    def volume_pointer(self):
+        p = C.c_void_p()
+        n = C.c_size_t()
+        call("etn_reconstruction_volume", self._h, C.byref(p), C.byref(n))
+        return p.value, n.value
+
+    def get_reconstructed_volume(self, dtype=None):
+        """Copy of the working volume as a device tensor [z][y][x]."""
+        torch = _torch()
+        tdt = {ETN_VOXEL_FLOAT32: torch.float32, ETN_VOXEL_FLOAT16: torch.float16,
+               ETN_VOXEL_BFLOAT16: torch.bfloat16}[self.voxel_type]
+        out = torch.empty(self.shape, dtype=dtype or tdt, device="cuda")
+        self.synchronize()
+        src, n = self.volume_pointer()
+        count = int(np.prod(self.shape))
+        call("etn_convert_volume", C.c_void_p(src), self.voxel_type, _ptr(out), _voxel_type_of(out), count,
+             _stream(None))
+        torch.cuda.synchronize()
+        return out
+
+    def set_volume(self, tensor):
+        if _voxel_type_of(tensor) != self.voxel_type or tuple(tensor.shape) != self.shape:
+            raise ValueError("volume must match the reconstruction's voxel type and shape")
+        torch = _torch()
+        torch.cuda.synchronize()
+        call("etn_reconstruction_set_volume", self._h, _ptr(tensor))
+        self.synchronize()

@@ -1,0 +1,88 @@
+// BlockIterativeReconstructionOperator: the SART / SIRT / block-iterative loop
+// (ettention/libReconstruction/src/algorithm/reconstruction/blockiterative/).
+#pragma once
+
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "operators.hpp"
+#include "projection_set.hpp"
+
+namespace etn {
+
+class FusedPlaneEngine;
+
+class BlockIterativeReconstructionOperator {
+public:
+    BlockIterativeReconstructionOperator(const etn_reconstruction_desc& desc, hipStream_t stream);
+    ~BlockIterativeReconstructionOperator();
+
+    // ReconstructionAlgorithm (ReconstructionAlgorithm.h:9-22)
+    void run();
+    void runSingleIteration();
+    void runNiterations(uint32_t iterations);
+    GpuVolume* getReconstructedVolume() { return volume_.get(); }
+    void setMaskVolume(const uint8_t* mask) { mask_.data = mask; invalidate_graph(); }
+    const uint8_t* getMaskVolume() const { return mask_.data; }
+
+    // Extension: ||A x - b||_2 over the used projections, FP64 accumulation (synchronises).
+    double residualNorm();
+    void synchronize();
+    hipStream_t stream() const { return stream_; }
+    bool usesFusedPath() const { return fused_ != nullptr; }
+    void invalidate_graph();
+
+    // BlockIterativeProjectionAccess (BlockIterativeProjectionAccess.cpp:11-110)
+    uint32_t GetBlockCount() const { return block_count_; }
+    uint32_t getBlockSize(uint32_t block) const;
+    uint32_t getPhysicalProjectionIndex(uint32_t block, uint32_t p) const;
+    const ParallelScannerGeometry& geometry(uint32_t physical) const { return geometries_.at(physical); }
+    const float* realProjection(uint32_t physical) const { return projections_ + (size_t)physical * pixels_; }
+
+private:
+    void enqueueIteration();
+    void handleOneIteration();
+    void doForwardProjectionForBlock(uint32_t block);
+    void computeResidualsForBlock(uint32_t block);
+    void doBackProjectionForBlock(uint32_t block);
+
+    etn_reconstruction_desc desc_;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    uint32_t iterations_;
+    float lambda_;
+    bool use_loc_;
+    uint32_t fp_samples_, bp_samples_;
+    uint32_t general_block_size_;
+    uint32_t block_count_ = 0;
+    std::unique_ptr<ProjectionSet> projection_set_;
+    std::vector<ParallelScannerGeometry> geometries_;     // per physical projection
+    const float* projections_;
+    size_t pixels_;
+
+    std::unique_ptr<GpuVolume> volume_;
+    GpuMask mask_;
+    std::unique_ptr<GeometricSetup> setup_;
+    std::unique_ptr<ForwardProjectionOperator> forward_;
+    std::unique_ptr<BackProjectionOperator> backward_;
+    std::unique_ptr<CompareKernel> compare_;
+    std::unique_ptr<VirtualProjectionKernel> loc_;
+
+    // BlockIterativeState: per-slot virtual projection, traversal length, residual
+    DeviceBuffer state_;
+    std::vector<DeviceImage> virtual_, traversal_, residual_;
+    std::vector<DeviceImage> real_;
+    std::vector<uint32_t> measured_;
+    uint32_t current_block_ = 0, current_projection_ = 0;
+
+    DeviceBuffer norm_scratch_;
+    std::unique_ptr<FusedPlaneEngine> fused_;
+
+    bool use_graph_;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+};
+
+}  // namespace etn

@@ -1,0 +1,234 @@
+// Device-side building blocks of the projectors: storage conversion, the DDA
+// ray traversal and the software bilinear lookup.  Every file including this
+// is compiled with -ffp-contract=off: the expressions below round exactly as
+// the OpenCL C source of the reference writes them, so the traversal takes
+// the same voxel sequence and the same segment lengths.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace etn {
+namespace dev {
+
+constexpr float kEpsilon = 1e-4f;     // ocl_src/Math.hl:4
+
+// ---- voxel storage ---------------------------------------------------------
+struct F32 {
+    using T = float;
+    static __device__ __forceinline__ float load(const float* p, size_t i) { return p[i]; }
+    static __device__ __forceinline__ float round(float f) { return f; }
+    static __device__ __forceinline__ void store(float* p, size_t i, float f) { p[i] = f; }
+};
+struct F16 {   // vload_half / vstore_half (round to nearest even), HalfFloatSupport.hl:6-9
+    using T = __half;
+    static __device__ __forceinline__ float load(const __half* p, size_t i) { return __half2float(p[i]); }
+    static __device__ __forceinline__ float round(float f) { return __half2float(__float2half_rn(f)); }
+    static __device__ __forceinline__ void store(__half* p, size_t i, float f) { p[i] = __float2half_rn(f); }
+};
+struct BF16 {  // bfloat16, round to nearest even
+    using T = uint16_t;
+    static __device__ __forceinline__ uint16_t pack(float f)
+    {
+        uint32_t u = __float_as_uint(f);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (uint16_t)(u >> 16);
+    }
+    static __device__ __forceinline__ float unpack(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+    static __device__ __forceinline__ float load(const uint16_t* p, size_t i) { return unpack(p[i]); }
+    static __device__ __forceinline__ float round(float f) { return unpack(pack(f)); }
+    static __device__ __forceinline__ void store(uint16_t* p, size_t i, float f) { p[i] = pack(f); }
+};
+
+// ---- DDA ray traversal (ocl_src/DDARaytracer.hl:20-110, RayUtils.hl:7-35) ---
+
+// Per-projection constants of the traversal.
+struct DdaConst {
+    float ray[3], inv[3], step[3];
+    int sign[3];
+};
+
+__device__ __forceinline__ DdaConst dda_const(const float ray[3], const float vs[3])
+{
+    DdaConst c;
+    for (int k = 0; k < 3; ++k) {
+        c.ray[k] = ray[k];
+        // invertVectorSafely (RayUtils.hl:25-35)
+        c.inv[k] = 1.0f / ray[k];
+        if (ray[k] == 0.0f) c.inv[k] = -FLT_MAX;
+        const float sg = ray[k] > 0.0f ? 1.0f : (ray[k] < 0.0f ? -1.0f : 0.0f);
+        c.step[k] = vs[k] * c.inv[k] * sg;    // stepDifference = voxelSize * invRay * sign(ray)
+        c.sign[k] = (int)sg;
+    }
+    return c;
+}
+
+// Traversal state of one ray; suspends at band boundaries and resumes.
+struct DdaRay {
+    int vox[3];
+    float res[3];
+    float acc, accl;
+    uint32_t brk;
+    int live;
+};
+
+// Entry into the subvolume box; leaves the ray dead when it misses
+// (getBoundingBoxEntryAndExitPointInRayCoordinates, doesRayIntersectBoundingBox,
+// calculateRayEntryPoint, getInitialVoxelIndex, getRemainingDifferenceToVoxelBorder).
+__device__ __forceinline__ void dda_begin(const SlabParams& s, const DdaConst& c, const float src[3], DdaRay& r)
+{
+    float tmin[3], tmax[3];
+    for (int k = 0; k < 3; ++k) {
+        const float t0 = (s.bmin[k] - src[k]) * c.inv[k];
+        const float t1 = (s.bmax[k] - src[k]) * c.inv[k];
+        tmin[k] = fminf(t0, t1);
+        tmax[k] = fmaxf(t0, t1);
+    }
+    const float t_in = fmaxf(fmaxf(tmin[0], tmin[1]), tmin[2]);
+    const float t_out = fminf(fminf(tmax[0], tmax[1]), tmax[2]);
+    r.acc = 0.0f;
+    r.accl = 0.0f;
+    r.brk = 0;
+    r.live = (t_out - t_in > kEpsilon) ? 1 : 0;
+    if (!r.live) {
+        r.vox[0] = r.vox[1] = r.vox[2] = -1;
+        r.res[0] = r.res[1] = r.res[2] = 0.0f;
+        return;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const float ivs = 1.0f / s.vs[k];
+        float p = src[k] + t_in * c.ray[k];
+        p = fminf(fmaxf(p, s.bmin[k]), s.bmax[k]);
+        int vi = (int)((p - s.bmin[k]) * ivs);      // convert_int4_rtz
+        vi = max(vi, 0);
+        vi = min(vi, s.res[k] - 1);
+        r.vox[k] = vi;
+        float d = floorf(p * ivs) * s.vs[k];
+        d = (c.ray[k] > 0.0f) ? (d + s.vs[k] - p) * c.inv[k] : (d - p) * c.inv[k];
+        d = (c.ray[k] == 0.0f) ? FLT_MAX : d;
+        d = (d == 0.0f) ? c.step[k] : d;
+        r.res[k] = d;
+    }
+}
+
+// Advances the ray while its voxel lies inside the subvolume and its z index
+// inside [zlo, zhi).  `sample(x, y, z, &value, &maskv)` reads the voxel.  The
+// loop condition is the reference's (in range, then ++breakVar < 4096, with a
+// second ++breakVar in the body: at most 2048 steps); a ray that leaves the
+// band with its condition not yet evaluated resumes unchanged.
+template <typename Sampler>
+__device__ __forceinline__ void dda_advance(const SlabParams& s, const DdaConst& c, DdaRay& r, int zlo, int zhi,
+                                            const Sampler& sample)
+{
+    while (r.live) {
+        if (!(r.vox[0] >= 0 && r.vox[1] >= 0 && r.vox[2] >= 0 && r.vox[0] < s.res[0] && r.vox[1] < s.res[1] &&
+              r.vox[2] < s.res[2])) {
+            r.live = 0;
+            break;
+        }
+        if (r.vox[2] < zlo || r.vox[2] >= zhi) break;
+        if (!(++r.brk < 4096u)) {
+            r.live = 0;
+            break;
+        }
+        const float dt = fminf(fminf(r.res[0], r.res[1]), r.res[2]);
+        float value, maskv;
+        sample(r.vox[0], r.vox[1], r.vox[2], value, maskv);
+        r.acc += dt * value;
+        r.accl += dt * maskv;
+        const int ix = (r.res[0] <= r.res[1] && r.res[0] <= r.res[2]) ? 1 : 0;
+        const int iy = (r.res[1] < r.res[0] && r.res[1] <= r.res[2]) ? 1 : 0;
+        const int iz = (r.res[2] < r.res[0] && r.res[2] < r.res[1]) ? 1 : 0;
+        r.res[0] = ix ? c.step[0] : r.res[0] - dt;
+        r.res[1] = iy ? c.step[1] : r.res[1] - dt;
+        r.res[2] = iz ? c.step[2] : r.res[2] - dt;
+        r.vox[0] += ix * c.sign[0];
+        r.vox[1] += iy * c.sign[1];
+        r.vox[2] += iz * c.sign[2];
+        ++r.brk;
+    }
+}
+
+// Sub-ray origin (ParallelBeamsForwardProjection.cl:12-24).
+__device__ __forceinline__ void fp_ray_source(const RayParams& g, uint32_t x, uint32_t y, uint32_t si, uint32_t sj,
+                                              float sd, float off, float src[3])
+{
+    const float fx = (float)x + off + sd * (float)si;
+    const float fy = (float)y + off + sd * (float)sj;
+    for (int k = 0; k < 3; ++k) {
+        const float ru = g.h[k] * fx;
+        const float rv = g.v[k] * fy;
+        src[k] = g.source[k] + ru + rv;
+    }
+}
+
+// ---- bilinear lookup with zero border (ocl_src/ImageHandler.hl:20-40) ------
+__device__ __forceinline__ float cl_fract(float x, float* base)
+{
+    const float fl = floorf(x);
+    *base = fl;
+    return fminf(x - fl, 0x1.fffffep-1f);
+}
+__device__ __forceinline__ float lerp_row(const float* row, int n, float c)
+{
+    float b;
+    const float t = cl_fract(c - 0.5f, &b);
+    const int i0 = (int)b, i1 = i0 + 1;
+    const float v0 = (i0 >= 0 && i0 < n) ? row[i0] : 0.0f;
+    const float v1 = (i1 >= 0 && i1 < n) ? row[i1] : 0.0f;
+    return v0 + (v1 - v0) * t;
+}
+__device__ __forceinline__ float bilinear(const float* img, int nu, int nv, float cx, float cy)
+{
+    float b;
+    const float t = cl_fract(cy - 0.5f, &b);
+    const int j0 = (int)b, j1 = j0 + 1;
+    const float v0 = (j0 >= 0 && j0 < nv) ? lerp_row(img + (size_t)j0 * nu, nu, cx) : 0.0f;
+    const float v1 = (j1 >= 0 && j1 < nv) ? lerp_row(img + (size_t)j1 * nu, nu, cx) : 0.0f;
+    return v0 + (v1 - v0) * t;
+}
+
+// Detector coordinate of a point (TransformCoordG, Matrix4x4.hl:38-58).
+__device__ __forceinline__ void project_point(const float* m, const float p[3], float& u, float& v)
+{
+    float r[4];
+    for (int row = 0; row < 4; ++row) {
+        float a = m[0 * 4 + row] * p[0] + m[1 * 4 + row] * p[1];
+        a = a + m[2 * 4 + row] * p[2];
+        r[row] = a + m[3 * 4 + row] * 1.0f;
+    }
+    u = r[0] / r[3];
+    v = r[1] / r[3];
+}
+
+// One voxel's correction: the sample loop of ParallelBeamsBackProjection.cl:26-44.
+// Returns the used-sample count; *sum receives the summed contributions.
+__device__ __forceinline__ uint32_t bp_voxel_sum(const BpParams& b, const float center[3], const float* residual,
+                                                 const float* ray_length, float* sum)
+{
+    float acc = 0.0f;
+    uint32_t used = 0;
+    for (uint32_t si = 0; si < b.nsamples; ++si) {
+        const float4 smp = b.samples[si];
+        const float p[3] = {center[0] + smp.x, center[1] + smp.y, center[2] + smp.z};
+        float u, v;
+        project_point(b.m, p, u, v);
+        const float L = bilinear(ray_length, (int)b.nu, (int)b.nv, u, v);
+        const float R = bilinear(residual, (int)b.nu, (int)b.nv, u, v);
+        const float w = smp.w / L;
+        const float contrib = R * b.lambda * w;
+        acc += (L > kEpsilon) ? contrib : 0.0f;
+        used += (L > kEpsilon) ? 1u : 0u;
+    }
+    *sum = acc;
+    return used;
+}
+
+}  // namespace dev
+}  // namespace etn

@@ -1,0 +1,56 @@
+// Parameter blocks passed by value to the projector kernels, and the host
+// launchers defined in projectors.hip.  Shared by host and device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "etn/etn.h"
+
+namespace etn {
+namespace dev {
+
+constexpr int kMaxBpSamples = 64;
+
+// VolumeGeometry of one subvolume (ocl_src/VolumeGeometry.hl:4-15) plus the
+// strides of the complete volume the subvolume lives in.
+struct SlabParams {
+    float bmin[3], bmax[3];     // subvolume bounding box
+    float vs[3];                // voxel size
+    int res[3];                 // subvolume resolution
+    uint32_t nx, ny;            // complete-volume strides
+    uint64_t first_voxel;       // linear index of the subvolume's first voxel
+};
+
+// ProjectionGeometry fields the FP reads (ocl_src/ProjectionGeometry.hl:4-15).
+struct RayParams {
+    float source[3], h[3], v[3], ray[3];
+    uint32_t nu, nv, samples;
+};
+
+// Inputs of ParallelBeamsBackProjection.cl:correct besides the volume.
+struct BpParams {
+    float m[16];                // projection matrix, column-major
+    float lambda;               // lambda / block size
+    uint32_t nu, nv, nsamples;
+    float4 samples[kMaxBpSamples];   // xyz offset, w = in-voxel chord length
+};
+
+void launch_forward_project(const SlabParams& s, const RayParams& r, etn_voxel_type vt, const void* volume,
+                            const uint8_t* mask, float* projection, float* traversal_length, hipStream_t stream);
+void launch_back_project(const SlabParams& s, const BpParams& b, etn_voxel_type vt, void* volume,
+                         const uint8_t* mask, const float* residual, const float* ray_length, hipStream_t stream);
+void launch_compare(uint32_t n, const float* real, const float* virt, float* residual, hipStream_t stream);
+void launch_long_object_compensate(uint32_t n, float* projection, float* length, float c, hipStream_t stream);
+void launch_ray_length(const float bmin[3], const float bmax[3], const RayParams& r, float* out, hipStream_t stream);
+// partials must hold ssd_partial_count() doubles
+constexpr uint32_t kSsdBlocks = 512;
+void launch_sum_squared_difference(const float* a, const float* b, size_t count, double* partials, double* out,
+                                   hipStream_t stream);
+void launch_convert(const void* src, etn_voxel_type st, void* dst, etn_voxel_type dt, size_t count, hipStream_t stream);
+void launch_fill(void* dst, etn_voxel_type t, float value, size_t count, hipStream_t stream);
+void launch_zero(float* dst, size_t count, hipStream_t stream);
+
+}  // namespace dev
+}  // namespace etn

@@ -1,0 +1,212 @@
+/*
+ * etn.h -- C ABI of libetn.so, the MI355X (gfx950) implementation of
+ * Ettention's block-iterative SART/SIRT hot path.
+ *
+ * Conventions
+ *   - Plain C types only.  Device pointers are HIP device addresses; `stream`
+ *     is a hipStream_t passed as void* (NULL = the null stream).
+ *   - Every function returns an etn_status; on failure etn_last_error()
+ *     returns a message (thread-local, valid until the next call on the thread).
+ *   - Kernel entry points are stream-ordered and asynchronous; they allocate
+ *     nothing on the device except where stated.
+ *   - Volumes are stored as in the reference: linear index z*ny*nx + y*nx + x
+ *     over the complete volume; a z-slab ("subvolume") is a contiguous run of
+ *     z slices (VolumeProperties.cpp:74-98).
+ *
+ * Reference paths are relative to the reference checkout root.
+ */
+#ifndef ETN_ETN_H
+#define ETN_ETN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum etn_status {
+    ETN_OK = 0,
+    ETN_ERROR_INVALID_ARGUMENT = 1, /* precondition the reference throws Exception for */
+    ETN_ERROR_UNSUPPORTED = 2,      /* no kernel variant for this geometry / basis */
+    ETN_ERROR_DEVICE = 3,           /* HIP runtime error (reference: GPUException / CL_ASSERT) */
+    ETN_ERROR_OUT_OF_MEMORY = 4,
+    ETN_ERROR_INTERNAL = 5
+} etn_status;
+
+/* Voxel storage (optimization.internalVoxelRepresentationType; Voxel.cpp:45-72).
+ * Arithmetic is FP32 for every storage type. */
+typedef enum etn_voxel_type {
+    ETN_VOXEL_FLOAT32 = 0,
+    ETN_VOXEL_FLOAT16 = 1,          /* IEEE half, round-to-nearest-even on store (vstore_half) */
+    ETN_VOXEL_BFLOAT16 = 2          /* extension: bfloat16, round-to-nearest-even on store */
+} etn_voxel_type;
+
+/* State of a ParallelScannerGeometry (ScannerGeometry.h; ScannerGeometry.cpp:95-103). */
+typedef struct etn_scanner_geometry {
+    float source[3];
+    float detector[3];
+    float horizontal_pitch[3];
+    float vertical_pitch[3];
+    uint32_t resolution[2];         /* detector (nu, nv) */
+} etn_scanner_geometry;
+
+/* VolumeProperties: resolution, complete bounding box, z-slab count
+ * (VolumeProperties.cpp:21-98; VolumeParameterSet.cpp:68-71). */
+typedef struct etn_volume_geometry {
+    uint32_t resolution[3];
+    float bbox_min[3];
+    float bbox_max[3];
+    uint32_t subvolume_count;       /* >= 1 */
+} etn_volume_geometry;
+
+#define ETN_ALL_SUBVOLUMES 0xffffffffu   /* BackProjectionOperator::ALL_SUB_VOLUMES */
+
+const char* etn_last_error(void);
+const char* etn_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Host-side geometry (no device work)
+ * ------------------------------------------------------------------------- */
+
+/* SingleAxisTiltRotator::createRotatedScannerGeometry
+ * (ettention/libReconstruction/src/model/geometry/SingleAxisTiltRotator.cpp:42-78). */
+etn_status etn_rotate_scanner_geometry(const etn_scanner_geometry* base, float tilt_deg,
+                                       float x_axis_tilt_deg, etn_scanner_geometry* out);
+
+/* ParallelScannerGeometry::getRayDirection / getProjectionMatrix (column-major,
+ * values[4*col+row]) / getTraversalLengthThroughBoundingBox
+ * (ettention/libReconstruction/src/model/geometry/ParallelScannerGeometry.cpp:44-69). */
+etn_status etn_ray_direction(const etn_scanner_geometry* g, float out[3]);
+etn_status etn_projection_matrix(const etn_scanner_geometry* g, float out[16]);
+etn_status etn_traversal_length_through_bbox(const etn_scanner_geometry* g, const float bbox_min[3],
+                                             const float bbox_max[3], float* out);
+
+/* ParallelBeamsBackProjectionKernel::generateSamplingPatternForPerVoxelSampling
+ * (ParallelBeamsBackProjectionKernel.cpp:69-103); out4 holds samples x (dx, dy, dz, l).
+ * Fails for samples that are zero or not a perfect square (:55-63). */
+etn_status etn_bp_sample_pattern(const etn_scanner_geometry* g, const float voxel_size[3],
+                                 uint32_t samples, float* out4);
+
+/* VolumeProperties::setSubVolumeCount / getSubVolume* (VolumeProperties.cpp:74-98). */
+etn_status etn_subvolume(const etn_volume_geometry* v, uint32_t index, uint32_t resolution_out[3],
+                         uint32_t* first_slice, float bbox_min_out[3], float bbox_max_out[3]);
+
+/* ProjectionIteratorFactory::instantiate + ProjectionSet::setProjectionCount/reorder:
+ * scheme "identity" | "random" | "maxangle" | "buckets"
+ * (ettention/libReconstruction/src/model/projectionset/factory/ProjectionIteratorFactory.cpp:11-23).
+ * `out` must hold `count` entries. */
+etn_status etn_projection_order(const char* scheme, uint32_t count, const uint32_t* skip,
+                                uint32_t skip_count, uint32_t* out, uint32_t* out_count);
+
+/* ---------------------------------------------------------------------------
+ * Kernels (device pointers, stream-ordered)
+ * ------------------------------------------------------------------------- */
+
+/* ParallelBeamsForwardProjectionKernel::run for one subvolume: DDA line
+ * integral of samples^2 sub-rays per pixel, ADDED to projection and
+ * traversal_length (ForwardProjectionOperator::Execute,
+ * ettention/libReconstruction/src/algorithm/projections/forward/ForwardProjectionOperator.cpp:31-50;
+ * ocl_src/ParallelBeamsForwardProjection.cl:26-58).  `volume` points at the
+ * complete volume; `mask` (uint8, 0..255, complete volume) may be NULL.
+ * samples 0 is treated as 1 (ForwardProjectionKernel.cpp:65-68). */
+etn_status etn_forward_project(const etn_volume_geometry* vol, etn_voxel_type voxel_type,
+                               const void* volume, const uint8_t* mask, uint32_t subvolume,
+                               const etn_scanner_geometry* g, uint32_t samples,
+                               float* projection, float* traversal_length, void* stream);
+
+/* BackProjectionOperator::setInput(residual, rayLength, blockSize) + execute(subvolume)
+ * (ettention/libReconstruction/src/algorithm/projections/back/BackProjectionOperator.cpp:43-98;
+ * ocl_src/ParallelBeamsBackProjection.cl:9-53): lambda is divided by block_size;
+ * subvolume may be ETN_ALL_SUBVOLUMES. */
+etn_status etn_back_project(const etn_volume_geometry* vol, etn_voxel_type voxel_type, void* volume,
+                            const uint8_t* mask, uint32_t subvolume, const etn_scanner_geometry* g,
+                            uint32_t samples, const float* residual, const float* ray_length,
+                            float lambda, uint32_t block_size, void* stream);
+
+/* CompareKernel: residual = real - virtual
+ * (ettention/libReconstruction/src/algorithm/imagemanipulation/CompareKernel.cpp:9-66; ocl_src/Compare.cl). */
+etn_status etn_compare(uint32_t nu, uint32_t nv, const float* real, const float* virtual_projection,
+                       float* residual, void* stream);
+
+/* VirtualProjectionKernel (long-object compensation): p = L > 1e-4 ? p*c/L : 0; L = c
+ * (ettention/libReconstruction/src/algorithm/longobjectcompensation/LongObjectCompensationKernel.cpp:9-59;
+ * ocl_src/VirtualProjection.cl). */
+etn_status etn_long_object_compensate(uint32_t nu, uint32_t nv, float* projection,
+                                      float* traversal_length, float constant_length, void* stream);
+
+/* ComputeRayLengthKernel: chord of each pixel-centre ray through the complete
+ * bounding box (ComputeRayLengthKernel.cpp:9-46; ocl_src/ComputeRayLength.cl). */
+etn_status etn_compute_ray_length(const etn_volume_geometry* vol, const etn_scanner_geometry* g,
+                                  float* ray_length, void* stream);
+
+/* Extension (no reference counterpart; the reference's only metric is the racy
+ * BufferRMS.cl): out_device[0] += sum_i (a_i - b_i)^2 accumulated in FP64, with b
+ * = NULL meaning zero.  Deterministic for a given count. */
+etn_status etn_sum_squared_difference_f64(const float* a, const float* b, size_t count,
+                                          double* out_device, void* stream);
+
+/* Storage conversion helpers for reduced-precision volumes (device to device). */
+etn_status etn_convert_volume(const void* src, etn_voxel_type src_type, void* dst,
+                              etn_voxel_type dst_type, size_t count, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Block-iterative reconstruction (BlockIterativeReconstructionOperator)
+ * ------------------------------------------------------------------------- */
+
+typedef struct etn_reconstruction etn_reconstruction;
+
+enum {
+    ETN_RECON_FLAG_GENERIC_KERNELS = 1u << 0,   /* disable the fused plane kernels */
+    ETN_RECON_FLAG_NO_GRAPH = 1u << 1           /* launch eagerly instead of replaying a hipGraph */
+};
+
+/* Parameters of BlockIterativeReconstructionOperator, named after the
+ * reference keys (AlgorithmParameterSet.cpp:8-53, OptimizationParameterSet.cpp:7-22,
+ * ForwardProjectionParameterSet, BackprojectionParameterSet, VolumeParameterSet,
+ * InputParameterSet, HardwareParameterSet). */
+typedef struct etn_reconstruction_desc {
+    const char* algorithm;              /* "sart" | "sirt" | "blockiterative" (case-insensitive) */
+    uint32_t block_size;                /* algorithm.blockSize (blockiterative only) */
+    uint32_t iterations;                /* algorithm.numberOfIterations (run) */
+    float lambda;                       /* algorithm.lambda */
+    int use_long_object_compensation;   /* algorithm.useLongObjectCompensation */
+    uint32_t fp_samples;                /* forwardProjection.samples */
+    uint32_t bp_samples;                /* backProjection.samples (perfect square) */
+    const char* projection_iteration;   /* optimization.projectionIteration; NULL = "random" */
+    const uint32_t* skip_projections;   /* input.skipProjections */
+    uint32_t skip_count;
+    etn_voxel_type voxel_type;          /* optimization.internalVoxelRepresentationType */
+    etn_volume_geometry volume;         /* volume.* + hardware.subVolumeCount (0 = 1 on this GPU) */
+    float init_value;                   /* volume.initValue */
+    etn_scanner_geometry base_geometry; /* untilted geometry of the projection stack */
+    uint32_t projection_count;
+    const float* tilt_angles;           /* host, degrees, projection_count entries */
+    float x_axis_tilt;                  /* input.xAxisTilt */
+    const float* projections;           /* device, projection_count * nu * nv, stack order */
+    const uint8_t* mask;                /* device prior-knowledge mask or NULL */
+    uint32_t flags;                     /* ETN_RECON_FLAG_* */
+} etn_reconstruction_desc;
+
+etn_status etn_reconstruction_create(const etn_reconstruction_desc* desc, void* stream,
+                                     etn_reconstruction** out);
+etn_status etn_reconstruction_destroy(etn_reconstruction* r);
+/* ReconstructionAlgorithm::run / runSingleIteration (ReconstructionAlgorithm.h:9-22);
+ * asynchronous on the reconstruction's stream. */
+etn_status etn_reconstruction_run(etn_reconstruction* r);
+etn_status etn_reconstruction_run_single_iteration(etn_reconstruction* r);
+etn_status etn_reconstruction_run_iterations(etn_reconstruction* r, uint32_t iterations);
+/* getReconstructedVolume: library-owned device buffer of the working volume. */
+etn_status etn_reconstruction_volume(etn_reconstruction* r, void** device_volume, size_t* bytes);
+/* Replace the working volume (volume.initFile): device source of the same type and size. */
+etn_status etn_reconstruction_set_volume(etn_reconstruction* r, const void* device_volume);
+/* ||A x - b||_2 over every used projection, FP64 accumulation; synchronises. */
+etn_status etn_reconstruction_residual_norm(etn_reconstruction* r, double* out);
+etn_status etn_reconstruction_synchronize(etn_reconstruction* r);
+/* 1 when the fused plane kernels run this reconstruction, 0 for the generic kernels. */
+etn_status etn_reconstruction_uses_fused_path(etn_reconstruction* r, int* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1073,3 +1073,16 @@ double orc_residual_norm(const orc_recon* r, const void* volume)
     free(virt); free(len); free(order);
     return sqrt(acc);
 }
+
+/* Worker threads the OpenMP loops above use (reported with the CPU baseline). */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -109,6 +109,7 @@ typedef struct orc_recon {
 int  orc_block_iterative(const orc_recon* r, void* volume, uint32_t iterations);
 int  orc_block_iterative_f64(const orc_recon* r, double* volume, uint32_t iterations);
 double orc_residual_norm(const orc_recon* r, const void* volume);
+int  orc_num_threads(void);
 
 #ifdef __cplusplus
 }

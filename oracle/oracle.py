@@ -120,6 +120,7 @@ def lib():
         L.orc_block_iterative_f64.restype = i32
         L.orc_residual_norm.argtypes = [C.POINTER(Recon), fp]
         L.orc_residual_norm.restype = C.c_double
+        L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -256,6 +257,10 @@ def projection_order(kind, count, skip=()):
     if lib().orc_projection_order(kind.encode(), int(count), _p(sk), len(skip), _p(out), _p(n)):
         raise ValueError("unknown projection iteration scheme: %s" % kind)
     return out[: int(n[0])].copy()
+
+
+def num_threads():
+    return int(lib().orc_num_threads())
 
 
 def mt19937_nth(seed, n):
