@@ -125,6 +125,8 @@ SIGNATURES = [
     ("etn_reconstruction_residual_norm", _i32, [_vp, C.POINTER(C.c_double)]),
     ("etn_reconstruction_synchronize", _i32, [_vp]),
     ("etn_reconstruction_uses_fused_path", _i32, [_vp, C.POINTER(C.c_int)]),
+    ("etn_reconstruction_time_kernels", _i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_uint32)]),
 ]
 
 _lib = None

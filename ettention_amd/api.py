@@ -228,6 +228,15 @@ class BlockIterativeReconstructionOperator:
         call("etn_reconstruction_uses_fused_path", self._h, C.byref(out))
         return bool(out.value)
 
+    def time_kernels(self):
+        """(kernel id, average ms, launches) of the dominant kernel over one extra
+        eagerly launched iteration (0 fused, 1 generic FP, 2 generic BP)."""
+        k = C.c_int()
+        ms = C.c_double()
+        n = C.c_uint32()
+        call("etn_reconstruction_time_kernels", self._h, C.byref(k), C.byref(ms), C.byref(n))
+        return k.value, ms.value, n.value
+
     def volume_pointer(self):
         p = C.c_void_p()
         n = C.c_size_t()

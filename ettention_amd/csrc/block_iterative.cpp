@@ -240,6 +240,66 @@ double BlockIterativeReconstructionOperator::residualNorm()
     return std::sqrt(host);
 }
 
-void BlockIterativeReconstructionOperator::synchronize() { ETN_HIP(hipStreamSynchronize(stream_)); }
+void BlockIterativeReconstructionOperator::synchronize()
+{
+    ETN_HIP(hipStreamSynchronize(stream_));
+    if (fused_) fused_->check(stream_);
+}
+
+double BlockIterativeReconstructionOperator::timeKernels(int* kernel, uint32_t* launches)
+{
+    if (fused_) {
+        *kernel = 0;
+        const double ms = fused_->timeKernels(stream_, launches);
+        fused_->check(stream_);
+        return ms;
+    }
+    // one eager iteration with an event pair around every FP and BP launch
+    std::vector<hipEvent_t> fp_ev, bp_ev;
+    auto timed = [&](std::vector<hipEvent_t>& evs, auto&& launch) {
+        hipEvent_t a, b;
+        ETN_HIP(hipEventCreate(&a));
+        ETN_HIP(hipEventCreate(&b));
+        ETN_HIP(hipEventRecord(a, stream_));
+        launch();
+        ETN_HIP(hipEventRecord(b, stream_));
+        evs.push_back(a);
+        evs.push_back(b);
+    };
+    const uint32_t subvolumes = volume_->properties().subvolume_count();
+    for (uint32_t block = 0; block < block_count_; ++block) {
+        const uint32_t bs = getBlockSize(block);
+        for (uint32_t p = 0; p < bs; ++p) dev::launch_zero(virtual_[p].data, 3 * pixels_, stream_);
+        for (uint32_t s = 0; s < subvolumes; ++s)
+            for (uint32_t p = 0; p < bs; ++p) {
+                setup_->setScannerGeometry(geometries_[getPhysicalProjectionIndex(block, p)]);
+                forward_->SetOutput(&virtual_[p], &traversal_[p]);
+                timed(fp_ev, [&] { forward_->Execute(s); });
+            }
+        computeResidualsForBlock(block);
+        for (uint32_t s = 0; s < subvolumes; ++s)
+            for (uint32_t p = 0; p < bs; ++p) {
+                setup_->setScannerGeometry(geometries_[getPhysicalProjectionIndex(block, p)]);
+                backward_->setInput(&residual_[p], &traversal_[p], bs);
+                timed(bp_ev, [&] { backward_->execute(subvolumes - s - 1); });
+            }
+    }
+    ETN_HIP(hipStreamSynchronize(stream_));
+    auto average = [](std::vector<hipEvent_t>& evs) {
+        double total = 0.0;
+        for (size_t i = 0; i < evs.size(); i += 2) {
+            float ms = 0.0f;
+            ETN_HIP(hipEventElapsedTime(&ms, evs[i], evs[i + 1]));
+            total += ms;
+        }
+        for (auto e : evs) (void)hipEventDestroy(e);
+        return evs.empty() ? 0.0 : total / (double)(evs.size() / 2);
+    };
+    const size_t n = fp_ev.size() / 2;
+    const double fp = average(fp_ev), bp = average(bp_ev);
+    *launches = (uint32_t)n;
+    *kernel = fp >= bp ? 1 : 2;
+    return fp >= bp ? fp : bp;
+}
 
 }  // namespace etn

@@ -30,6 +30,9 @@ public:
     // Extension: ||A x - b||_2 over the used projections, FP64 accumulation (synchronises).
     double residualNorm();
     void synchronize();
+    // One eager iteration with HIP events around every launch; returns the
+    // average milliseconds of the dominant kernel (0 fused, 1 FP, 2 BP).
+    double timeKernels(int* kernel, uint32_t* launches);
     hipStream_t stream() const { return stream_; }
     bool usesFusedPath() const { return fused_ != nullptr; }
     void invalidate_graph();

@@ -338,4 +338,15 @@ etn_status etn_reconstruction_uses_fused_path(etn_reconstruction* r, int* out)
     });
 }
 
+etn_status etn_reconstruction_time_kernels(etn_reconstruction* r, int* kernel, double* average_ms, uint32_t* launches)
+{
+    return guarded([&] {
+        need(r, "reconstruction");
+        need(kernel, "kernel");
+        need(average_ms, "average_ms");
+        need(launches, "launches");
+        *average_ms = r->op->timeKernels(kernel, launches);
+    });
+}
+
 }  // extern "C"

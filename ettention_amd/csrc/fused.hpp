@@ -20,6 +20,11 @@ public:
                                                     const etn_reconstruction_desc& desc, hipStream_t stream);
     virtual ~FusedPlaneEngine() = default;
     virtual void enqueueIteration(hipStream_t stream) = 0;
+    // Throws when a kernel found a live ray outside the plane its row was assigned to.
+    virtual void check(hipStream_t stream) = 0;
+    // One eager iteration with events around every launch: average milliseconds of
+    // the launches that carry both a back and a forward projection.
+    virtual double timeKernels(hipStream_t stream, uint32_t* launches) = 0;
 };
 
 }  // namespace etn

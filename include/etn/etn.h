@@ -205,6 +205,14 @@ etn_status etn_reconstruction_residual_norm(etn_reconstruction* r, double* out);
 etn_status etn_reconstruction_synchronize(etn_reconstruction* r);
 /* 1 when the fused plane kernels run this reconstruction, 0 for the generic kernels. */
 etn_status etn_reconstruction_uses_fused_path(etn_reconstruction* r, int* out);
+/* Extension (profiling; the reference's CL-event PerformanceReport,
+ * CLAbstractionLayer.cpp:295-329): runs ONE more iteration eagerly with HIP
+ * events around every launch on the reconstruction's stream and reports the
+ * average duration of the kernel that dominates it: *kernel = 0 fused plane
+ * kernel (launches carrying a back and a forward projection), 1 generic forward
+ * projection, 2 generic back projection.  Advances the reconstruction. */
+etn_status etn_reconstruction_time_kernels(etn_reconstruction* r, int* kernel, double* average_ms,
+                                           uint32_t* launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,140 @@
+"""The fused plane kernels (BP of block b-1 fused with the FP of block b, one
+workgroup per volume plane) against the generic kernels and the oracle.
+
+The fused kernels repeat the generic kernels' float operations for every ray
+and voxel, so the volumes must be bit-identical; against the oracle the bound
+is the FP32 parity bound, relative L2 <= 1e-5."""
+import numpy as np
+import pytest
+
+import ettention_amd as E
+from ettention_amd import api as A
+from ettention_amd import synthetic as S
+from oracle import oracle as O
+from tests._util import load_golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TEST_BASE = E.ScannerGeometry.make((-64, -64, -128), (-64, -64, 128), (0.5, 0, 0), (0, 0.5, 0), (256, 256))
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def og(g):
+    return O.Scanner.make(g.source, g.detector, g.horizontal_pitch, g.vertical_pitch, g.resolution)
+
+
+def ov(v):
+    return O.Volume.make(tuple(v.resolution), tuple(v.bbox_min), tuple(v.bbox_max), v.subvolume_count)
+
+
+def run_pair(vg, base, tilts, projs_t, iterations, **kw):
+    out = []
+    for flags in (0, E.ETN_RECON_FLAG_GENERIC_KERNELS):
+        r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                                   projections=projs_t, iterations=iterations, flags=flags, **kw)
+        assert r.uses_fused_path() == (flags == 0)
+        r.run()
+        r.synchronize()
+        out.append((r, r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy()))
+    return out
+
+
+@pytest.fixture(scope="module")
+def synthetic():
+    s = load_golden("synthetic_3_parallel")
+    return np.ascontiguousarray(s["ground_truth"]), s["angles"], np.ascontiguousarray(s["projections"])
+
+
+@pytest.mark.parametrize("case", [
+    dict(algorithm="sart", lam=0.1, projection_iteration="random"),
+    dict(algorithm="blockiterative", block_size=10, lam=1.0, projection_iteration="maxangle",
+         use_long_object_compensation=True),
+    dict(algorithm="sirt", lam=1.0, projection_iteration="identity"),
+    dict(algorithm="blockiterative", block_size=7, lam=0.5, projection_iteration="buckets", skip_projections=(3, 90)),
+    dict(algorithm="sart", lam=0.3, projection_iteration="identity", voxel_type="half"),
+    dict(algorithm="blockiterative", block_size=3, lam=0.3, projection_iteration="random", voxel_type="bfloat16"),
+])
+def test_fused_equals_generic_on_reference_dataset(synthetic, case):
+    # two detector rows per volume plane (pitch 0.5), tilts -90..89 (rays change z sign)
+    gt, tilts, projs = synthetic
+    sub = slice(None, None, 3)
+    vg = E.VolumeGeometry.make((64, 64, 64))
+    (rf, fused), (rg, generic) = run_pair(vg, TEST_BASE, tilts[sub], dev(projs[sub]), 2, **case)
+    np.testing.assert_array_equal(fused, generic)
+    assert rf.residual_norm() == rg.residual_norm()
+    if case.get("voxel_type", "float") == "float":
+        B = {"sart": 1, "sirt": 0xFFFFFFFF}.get(case["algorithm"], case.get("block_size", 1))
+        spec = O.ReconSpec(ov(vg), og(TEST_BASE), tilts[sub], projs[sub], block_size=B,
+                           order=case["projection_iteration"], lam=case["lam"],
+                           use_loc=case.get("use_long_object_compensation", False),
+                           skip=case.get("skip_projections", ()))
+        ref = spec.run(np.zeros((64, 64, 64), np.float32), 2)
+        assert rel_l2(fused, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("shape,det", [((64, 48, 40), (64, 48)), ((96, 80, 72), (96, 80)), ((130, 66, 33), (130, 66))])
+def test_fused_equals_generic_on_mrc_geometry(shape, det):
+    # MRC stack geometry (unit pitch, one detector row per plane), odd sizes
+    nx, ny, nz = shape
+    truth = S.ellipsoid_phantom((nz, ny, nx), seed=77)
+    vg = E.VolumeGeometry.make(shape)
+    base = E.ScannerGeometry.mrc_base(*det)
+    tilts = S.tilt_series(60.0, 4.0)
+    truth_t = dev(truth)
+    projs = torch.zeros((len(tilts), det[1], det[0]), dtype=torch.float32, device="cuda")
+    scratch = torch.zeros((det[1], det[0]), dtype=torch.float32, device="cuda")
+    for i, t in enumerate(tilts):
+        A.forward_project(vg, truth_t, A.rotate(base, float(t)), projs[i], scratch)
+    (rf, fused), (rg, generic) = run_pair(vg, base, tilts, projs, 3, algorithm="sart", lam=0.3,
+                                          projection_iteration="random")
+    np.testing.assert_array_equal(fused, generic)
+    spec = O.ReconSpec(ov(vg), og(base), tilts, projs.cpu().numpy(), block_size=1, order="random", lam=0.3)
+    ref = spec.run(np.zeros((nz, ny, nx), np.float32), 3)
+    assert rel_l2(fused, ref) <= 1e-5
+
+
+def test_ineligible_configurations_use_generic_kernels(synthetic):
+    gt, tilts, projs = synthetic
+    sub = slice(None, None, 20)
+    p = dev(projs[sub])
+    for kw in (dict(fp_samples=4), dict(bp_samples=4), dict(volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=2)),
+               dict(x_axis_tilt=5.0), dict(mask=torch.full((64, 64, 64), 255, dtype=torch.uint8, device="cuda"))):
+        args = dict(volume=E.VolumeGeometry.make((64, 64, 64)), base_geometry=TEST_BASE, tilt_angles=tilts[sub],
+                    projections=p, algorithm="sart")
+        args.update(kw)
+        r = A.BlockIterativeReconstructionOperator(**args)
+        assert not r.uses_fused_path(), kw
+
+
+@pytest.mark.parametrize("workload", [
+    dict(res=(512, 512, 512), algorithm="sart", block_size=0),
+    dict(res=(512, 512, 256), algorithm="blockiterative", block_size=8),
+])
+def test_full_size_fused_equals_generic(workload):
+    # the benchmark shapes: 512^3 SART and 512x512x256 block-iterative B=8, 121 tilts
+    nx, ny, nz = workload["res"]
+    vg = E.VolumeGeometry.make((nx, ny, nz))
+    base = E.ScannerGeometry.mrc_base(512, 512)
+    tilts = S.tilt_series(60.0, 1.0)
+    phantom = S.ellipsoid_phantom((nz, ny, nx), seed=1001, device="cuda")
+    projs = torch.zeros((len(tilts), 512, 512), dtype=torch.float32, device="cuda")
+    scratch = torch.zeros((512, 512), dtype=torch.float32, device="cuda")
+    for i, t in enumerate(tilts):
+        A.forward_project(vg, phantom, A.rotate(base, float(t)), projs[i], scratch)
+    del phantom
+    vols = []
+    for flags in (0, E.ETN_RECON_FLAG_GENERIC_KERNELS):
+        r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                                   projections=projs, algorithm=workload["algorithm"],
+                                                   block_size=workload["block_size"], lam=0.3,
+                                                   projection_iteration="random", flags=flags)
+        assert r.uses_fused_path() == (flags == 0)
+        r.run_single_iteration()
+        r.synchronize()
+        vols.append(r.get_reconstructed_volume())
+        del r
+    assert torch.equal(vols[0], vols[1])
