@@ -72,43 +72,68 @@ struct FusedVolume {
 
 namespace {
 
-// DDA state of a ray confined to plane y = j: res.y is FLT_MAX and the y index
-// never steps, so only the x/z parts are kept.  With res.y == FLT_MAX the
-// reference's step choice reduces exactly to the comparisons below.
+// DDA state of a ray confined to plane y = j.  The y index never steps and
+// res.y stays FLT_MAX, which turns the reference's step choice
+//   ix = rx <= ry && rx <= rz,  iz = rz < rx && rz < ry,  dt = min(rx, ry, rz)
+// into exactly ix = rx <= rz, iz = !ix, dt = min(rx, rz): rx and rz are never
+// NaN and never above FLT_MAX (an axis with zero direction keeps FLT_MAX and is
+// never chosen; every other value is a finite difference).
 struct PlaneRay {
     int vx, vz;
     float rx, rz;
     float acc, accl;
     uint32_t brk;
     int live;
+    float stepx, stepz;
+    int sx, sz;
 };
 
-template <typename S>
-__device__ __forceinline__ void plane_advance(PlaneRay& r, const DdaConst& c, const float* band, int nx, int nz,
-                                              int zlo, int zhi)
+// Advances a ray while it is inside the volume and inside band [zlo, zhi).  A
+// ray leaving the volume is finished; one leaving the band resumes in the next
+// band with unchanged state.  kCap keeps the reference's 2048-step limit; it is
+// dropped when nx + nz <= 2049, since every step moves vx or vz monotonically by
+// one, so a ray can take at most nx + nz - 1 steps inside the volume.
+template <bool kCap>
+__device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int nx, int nz, int zlo, int zhi)
 {
-    while (r.live) {
-        if (!(r.vx >= 0 && r.vz >= 0 && r.vx < nx && r.vz < nz)) {
-            r.live = 0;
+    if (!r.live) return;
+    const unsigned h = (unsigned)(zhi - zlo);
+    const int lz = r.sz * nx;
+    int li = (r.vz - zlo) * nx + r.vx;
+    int vx = r.vx, vz = r.vz;
+    float rx = r.rx, rz = r.rz, acc = r.acc, accl = r.accl;
+    for (;;) {
+        const bool inx = (unsigned)vx < (unsigned)nx;
+        if (!(inx && (unsigned)(vz - zlo) < h)) {
+            if (!inx || (unsigned)vz >= (unsigned)nz) r.live = 0;
             break;
         }
-        if (r.vz < zlo || r.vz >= zhi) break;
-        if (!(++r.brk < 4096u)) {
-            r.live = 0;
-            break;
+        if (kCap) {
+            if (!(++r.brk < 4096u)) {
+                r.live = 0;
+                break;
+            }
         }
-        const float dt = fminf(fminf(r.rx, FLT_MAX), r.rz);
-        const float value = band[(r.vz - zlo) * nx + r.vx];
-        r.acc += dt * value;
-        r.accl += dt * 1.0f;
-        const int ix = (r.rx <= FLT_MAX && r.rx <= r.rz) ? 1 : 0;
-        const int iz = (r.rz < r.rx && r.rz < FLT_MAX) ? 1 : 0;
-        r.rx = ix ? c.step[0] : r.rx - dt;
-        r.rz = iz ? c.step[2] : r.rz - dt;
-        r.vx += ix * c.sign[0];
-        r.vz += iz * c.sign[2];
-        ++r.brk;
+        // dt = min(rx, rz) as a compare-select: it differs from fminf at most in
+        // the sign of a zero, which changes neither the sums nor the next res values
+        const bool ix = rx <= rz;
+        const float dt = ix ? rx : rz;
+        acc += dt * band[li];
+        accl += dt;                                     // dt * 1.0f
+        const float rest = (ix ? rz : rx) - dt;         // the axis not stepped loses dt
+        rx = ix ? r.stepx : rest;
+        rz = ix ? rest : r.stepz;
+        vx += ix ? r.sx : 0;
+        vz += ix ? 0 : r.sz;
+        li += ix ? r.sx : lz;
+        if (kCap) ++r.brk;
     }
+    r.vx = vx;
+    r.vz = vz;
+    r.rx = rx;
+    r.rz = rz;
+    r.acc = acc;
+    r.accl = accl;
 }
 
 // Four-voxel (or one-voxel) groups of a band row in the storage type: HBM
@@ -163,16 +188,13 @@ constexpr int kLdsFloats = 16384;   // 64 KiB per workgroup
 constexpr int kRowPad = 2;          // zero taps on each side of a staged detector row
 
 // LinearInterpolation (ImageHandler.hl:20-29) on a staged row with kRowPad zeros on
-// each side: clamping the base index to [-2, n] reads only zeros wherever the
-// reference's range checks return 0, so the value is the same.
-__device__ __forceinline__ float lerp_padded(const float* row, int n, int i0, float t)
-{
-    const int i = min(max(i0, -kRowPad), n) + kRowPad;
-    const float v0 = row[i], v1 = row[i + 1];
-    return v0 + (v1 - v0) * t;
-}
+// each side, stored as (v[i], v[i+1] - v[i]) pairs: clamping the base index to
+// [-2, n] reads only zeros wherever the reference's range checks return 0, and
+// the stored difference is the subtraction mix() performs, so the value is the same.
+__device__ __forceinline__ int padded_index(int i0, int n) { return min(max(i0, -kRowPad), n) + kRowPad; }
+__device__ __forceinline__ float lerp_pair(float2 p, float t) { return p.x + p.y * t; }
 
-template <typename S, bool kVec, int RPT>
+template <typename S, bool kVec, bool kCap, int RPT>
 __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume fv, const FusedPass* __restrict__ pass)
 {
     using IO = GroupIo<S, kVec>;
@@ -184,7 +206,8 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
     const int nx = fv.nx, nz = fv.nz, nu = fv.nu;
     const int row_len = nu + 2 * kRowPad;
     float* band = lds;                                  // band x nx
-    float* rows = lds + fv.band * nx;                   // 4 padded rows: residual j0, j0+1, length j0, j0+1
+    // 4 padded detector rows of (value, difference), 8-byte aligned after the band
+    float2* rows = reinterpret_cast<float2*>(lds + ((fv.band * nx + 1) & ~1));
     typename S::T* vol = static_cast<typename S::T*>(fv.data);
     const size_t plane_stride = (size_t)nx * fv.ny;
     const size_t plane_base = (size_t)j * nx;
@@ -232,6 +255,10 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
             ray[k].rz = r.res[2];
             ray[k].brk = r.brk;
             ray[k].live = r.live;
+            ray[k].stepx = c.step[0];
+            ray[k].stepz = c.step[2];
+            ray[k].sx = c.sign[0];
+            ray[k].sz = c.sign[2];
         }
     }
 
@@ -253,9 +280,14 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
         for (int e = tid; e < 2 * row_len; e += kFusedThreads) {
             const int rr = e / row_len, k = e - rr * row_len, u = k - kRowPad;
             const int row = j0 + rr;
-            const bool ok = row >= 0 && row < fv.nv && u >= 0 && u < nu;
-            rows[rr * row_len + k] = ok ? bp.residual[(size_t)row * nu + u] : 0.0f;
-            rows[(2 + rr) * row_len + k] = ok ? bp.length[(size_t)row * nu + u] : 0.0f;
+            const bool rok = row >= 0 && row < fv.nv;
+            const bool ok0 = rok && u >= 0 && u < nu, ok1 = rok && u + 1 >= 0 && u + 1 < nu;
+            const float r0 = ok0 ? bp.residual[(size_t)row * nu + u] : 0.0f;
+            const float r1 = ok1 ? bp.residual[(size_t)row * nu + u + 1] : 0.0f;
+            const float l0 = ok0 ? bp.length[(size_t)row * nu + u] : 0.0f;
+            const float l1 = ok1 ? bp.length[(size_t)row * nu + u + 1] : 0.0f;
+            rows[rr * row_len + k] = make_float2(r0, r1 - r0);
+            rows[(2 + rr) * row_len + k] = make_float2(l0, l1 - l0);
         }
     };
 
@@ -327,44 +359,70 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
                 __syncthreads();
             }
             const FusedBp& bp = pass->bp[q];
-            const float* r0 = rows;
-            const float* r1 = rows + row_len;
-            const float* l0 = rows + 2 * row_len;
-            const float* l1 = rows + 3 * row_len;
+            const float2* r0 = rows;
+            const float2* r1 = rows + row_len;
+            const float2* l0 = rows + 2 * row_len;
+            const float2* l1 = rows + 3 * row_len;
+            const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda, chord = bp.sample.w;
+            const float a4 = bp.m[4] * (py + bp.sample.y);
+            // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
+            auto x_term = [&](int x) {
+                const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + bp.sample.x;
+                return m0 * px + a4;
+            };
+            auto z_term = [&](int zz) {
+                const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + bp.sample.z;
+                return m8 * pz;
+            };
+            // ParallelBeamsBackProjection.cl:26-52 for one sample
+            auto update = [&](float xt, float zt, float value) {
+                float u = xt + zt;
+                u = u + m12 * 1.0f;
+                float bu;
+                const float tx = cl_fract(u - 0.5f, &bu);
+                const int i = padded_index((int)bu, nu);
+                const float L0 = lerp_pair(l0[i], tx), L1 = lerp_pair(l1[i], tx);
+                const float L = L0 + (L1 - L0) * ty;
+                const float R0 = lerp_pair(r0[i], tx), R1 = lerp_pair(r1[i], tx);
+                const float R = R0 + (R1 - R0) * ty;
+                const float wgt = chord / L;
+                const float contrib = R * lam * wgt;
+                const float sum = 0.0f + contrib;             // sum over the one sample
+                value = (L > kEpsilon) ? value + sum : value; // += 1 * sum / 1 when used
+                return S::round(value);
+            };
+            const int nrow = z1 - z0;
+            if (kFusedThreads % gpr == 0) {
+                // every thread keeps the same x group in every band row
+                const int x0 = (tid % gpr) * G, dz = kFusedThreads / gpr;
+                float xt[G];
+#pragma unroll
+                for (int i = 0; i < G; ++i) xt[i] = x_term(x0 + i);
 #pragma unroll 1
-            for (int gi = tid; gi < ng; gi += kFusedThreads) {
-                {
-                    const int zz = gi / gpr, xg = gi - zz * gpr;
-                    const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + bp.sample.z;
+                for (int zz = tid / gpr; zz < nrow; zz += dz) {
+                    const float zt = z_term(zz);
                     float out[G];
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
-                        const int x = xg * G + i;
-                        const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + bp.sample.x;
-                        float u = bp.m[0] * px + bp.m[4] * (py + bp.sample.y);
-                        u = u + bp.m[8] * pz;
-                        u = u + bp.m[12] * 1.0f;               // w == 1 exactly (checked on the host)
-                        float bu;
-                        const float tx = cl_fract(u - 0.5f, &bu);
-                        const int i0 = (int)bu;
-                        const float L0 = lerp_padded(l0, nu, i0, tx);
-                        const float L1 = lerp_padded(l1, nu, i0, tx);
-                        const float L = L0 + (L1 - L0) * ty;
-                        const float R0 = lerp_padded(r0, nu, i0, tx);
-                        const float R1 = lerp_padded(r1, nu, i0, tx);
-                        const float R = R0 + (R1 - R0) * ty;
-                        const float wgt = bp.sample.w / L;
-                        const float contrib = R * bp.lambda * wgt;
-                        float sum = 0.0f;
-                        sum += (L > kEpsilon) ? contrib : 0.0f;
-                        float value = band[zz * nx + x];
-                        if (L > kEpsilon) value += 1.0f * sum / 1.0f;
-                        value = S::round(value);
-                        band[zz * nx + x] = value;
-                        out[i] = value;
+                        out[i] = update(xt[i], zt, band[zz * nx + x0 + i]);
+                        band[zz * nx + x0 + i] = out[i];
                     }
                     if (q == nbp - 1)
-                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G, out);
+                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)x0, out);
+                }
+            } else {
+#pragma unroll 1
+                for (int gi = tid; gi < ng; gi += kFusedThreads) {
+                    const int zz = gi / gpr, x0 = (gi - zz * gpr) * G;
+                    const float zt = z_term(zz);
+                    float out[G];
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        out[i] = update(x_term(x0 + i), zt, band[zz * nx + x0 + i]);
+                        band[zz * nx + x0 + i] = out[i];
+                    }
+                    if (q == nbp - 1)
+                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)x0, out);
                 }
             }
             if (nbp > 1) __syncthreads();
@@ -376,11 +434,7 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 const int id = k * kFusedThreads + tid;
-                if (id < nrays && ray[k].live) {
-                    const FusedFp& f = pass->fp[id / rays_per_proj];
-                    const DdaConst c = dda_const(f.ray, s.vs);
-                    plane_advance<S>(ray[k], c, band, nx, nz, z0, z1);
-                }
+                if (id < nrays) plane_advance<kCap>(ray[k], band, nx, nz, z0, z1);
             }
         }
         __syncthreads();
@@ -431,23 +485,27 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
     }
 }
 
-template <typename S, bool kVec>
-void launch_fused_vec(int rpt, const FusedVolume& fv, const FusedPass* pass, size_t lds, hipStream_t stream)
+template <typename S, bool kVec, bool kCap>
+void launch_fused_cap(int rpt, const FusedVolume& fv, const FusedPass* pass, size_t lds, hipStream_t stream)
 {
     const dim3 grid(fv.ny), block(kFusedThreads);
     switch (rpt) {
-        case 1: fused_plane_kernel<S, kVec, 1><<<grid, block, lds, stream>>>(fv, pass); break;
-        case 2: fused_plane_kernel<S, kVec, 2><<<grid, block, lds, stream>>>(fv, pass); break;
-        case 4: fused_plane_kernel<S, kVec, 4><<<grid, block, lds, stream>>>(fv, pass); break;
-        default: fused_plane_kernel<S, kVec, 8><<<grid, block, lds, stream>>>(fv, pass); break;
+        case 1: fused_plane_kernel<S, kVec, kCap, 1><<<grid, block, lds, stream>>>(fv, pass); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, 2><<<grid, block, lds, stream>>>(fv, pass); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, 4><<<grid, block, lds, stream>>>(fv, pass); break;
+        default: fused_plane_kernel<S, kVec, kCap, 8><<<grid, block, lds, stream>>>(fv, pass); break;
     }
 }
 
 template <typename S>
 void launch_fused_typed(int rpt, const FusedVolume& fv, const FusedPass* pass, size_t lds, hipStream_t stream)
 {
-    if (fv.nx % 4 == 0) launch_fused_vec<S, true>(rpt, fv, pass, lds, stream);
-    else launch_fused_vec<S, false>(rpt, fv, pass, lds, stream);
+    const bool vec = fv.nx % 4 == 0;
+    const bool cap = fv.nx + fv.nz > 2049;
+    if (vec && !cap) launch_fused_cap<S, true, false>(rpt, fv, pass, lds, stream);
+    else if (vec) launch_fused_cap<S, true, true>(rpt, fv, pass, lds, stream);
+    else if (!cap) launch_fused_cap<S, false, false>(rpt, fv, pass, lds, stream);
+    else launch_fused_cap<S, false, true>(rpt, fv, pass, lds, stream);
 }
 
 }  // namespace
@@ -512,7 +570,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
     if (d.mask) return nullptr;
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
     const int nu = (int)d.base_geometry.resolution[0], nv = (int)d.base_geometry.resolution[1];
-    if (4 * (nu + 2 * dev::kRowPad) + nx > dev::kLdsFloats) return nullptr;   // a z row and the detector rows fit in LDS
+    if (8 * (nu + 2 * dev::kRowPad) + nx + 1 > dev::kLdsFloats) return nullptr;   // a z row and the detector rows fit in LDS
 
     // every projection: rays in y planes, constant detector row per plane
     std::vector<int> plane_of_row;
@@ -521,6 +579,10 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
             const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
             const Vec3f r = g.ray_direction();
             if (r.y != 0.0f || g.horizontal_pitch().y != 0.0f) return nullptr;
+            // nonzero x/z components must have finite reciprocals (the plane
+            // DDA relies on every res value staying at or below FLT_MAX)
+            for (float c : {r.x, r.z})
+                if (c != 0.0f && std::fabs(c) < 1e-30f) return nullptr;
             const Matrix4x4 m = g.projection_matrix();
             const float* mv = m.data();
             if (mv[1] != 0.0f || mv[9] != 0.0f || mv[3] != 0.0f || mv[7] != 0.0f || mv[11] != 0.0f || mv[15] != 1.0f)
@@ -606,9 +668,9 @@ FusedPlaneEngineImpl::FusedPlaneEngineImpl(BlockIterativeReconstructionOperator&
     fv_.ndead = (int)dead_rows.size();
     fv_.diag = diag_.as<int>();
     // 64 KiB of LDS per workgroup: 4 padded detector rows, the rest a band of z rows
-    const int row_floats = 4 * (nu_ + 2 * dev::kRowPad);
-    fv_.band = std::max(1, std::min(fv_.nz, (dev::kLdsFloats - row_floats) / fv_.nx));
-    lds_bytes_ = sizeof(float) * ((size_t)fv_.band * fv_.nx + (size_t)row_floats);
+    const int row_floats = 8 * (nu_ + 2 * dev::kRowPad);   // 4 rows of (value, difference)
+    fv_.band = std::max(1, std::min(fv_.nz, (dev::kLdsFloats - row_floats - 1) / fv_.nx));
+    lds_bytes_ = sizeof(float) * ((((size_t)fv_.band * fv_.nx + 1) & ~(size_t)1) + (size_t)row_floats);
     max_fp_ = std::max(1, std::min(dev::kMaxFusedFp,
                                    dev::kFusedThreads * dev::kMaxRaysPerThread / std::max(1, rows_per_plane_ * nu_)));
     build_passes();
