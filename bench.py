@@ -49,7 +49,8 @@ def main():
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--generic", action="store_true", help="force the generic (unfused) kernels")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-projections", type=int, default=4)
+    ap.add_argument("--cpu-sample-projections", type=int, default=32,
+                    help="projection steps the CPU baseline times (about 0.35 s each on 16 threads)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (traffic calibration for rocprof counters)")
     args = ap.parse_args()
@@ -177,10 +178,25 @@ def kernel_roofline(r, voxel_type, fused, V, P):
     achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
     log("dominant kernel %s: %.4f ms average over %d launches -> %.1f GB/s algorithmic"
         % (name, dur_ms, launches, achieved))
+    traffic, source = measured_traffic(kernel, voxel_type)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source, "kernel": name,
             "avg_launch_ms": round(dur_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes),
             "kernel_compulsory_bytes_per_launch": int(kernel_bytes), "timed_launches": launches}
+
+
+def measured_traffic(kernel, voxel_type):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), when one exists for it."""
+    name = {0: "fused_plane_kernel", 1: "forward_project_kernel", 2: "back_project_kernel"}[kernel]
+    vt = {"float": "f32", "half": "f16", "bfloat16": "bf16"}[voxel_type]
+    path = os.path.join(ROOT, "profiles", "%s_sart_512cubed_121_%s_traffic.md" % (name, vt))
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(wl, tilts, projs, nsample):
