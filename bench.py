@@ -1,7 +1,7 @@
 """SART throughput on one MI355X: GUPS on the 512^3 x 121-tilt workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxel-type float|half|bfloat16]
-                    [--workload metric|c2] [--generic] [--no-cpu-baseline]
+                    [--workload metric|c2|c4] [--generic] [--no-cpu-baseline]
 
 A step is one full iteration of the block-iterative loop over the synthetic
 tilt series (for SART: every projection's forward projection, residual and
@@ -33,6 +33,10 @@ WORKLOADS = {
     "c2": dict(res=(512, 512, 256), det=(512, 512), max_angle=60.0, step=1.0, algorithm="blockiterative",
                block_size=8, order="random", lam=0.3, beads=200,
                name="block-iterative B=8 on 512x512x256 + 200 beads, 121 tilts +-60 deg by 1 deg, 512^2 detector"),
+    # 64 independent SART reconstructions advanced together
+    "c4": dict(members=64, algorithm="sart", order="random", lam=0.3,
+               name="batch of 64 SART reconstructions, cube edges uniform over {64,96,128,192,256}, "
+                    "tilt counts uniform over [41,121] at +-60 deg, detector = edge^2"),
 }
 
 
@@ -59,6 +63,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the volume shards along y with "
               "no exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c4":
+        run_batch(args)
+        return
 
     import torch
     import ettention_amd as E
@@ -149,6 +157,90 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_batch(args):
+    """The batched workload: GUPS = sum over members of voxels x projections,
+    per iteration, over the wall time of the iterations."""
+    import torch
+    import ettention_amd as E
+    from ettention_amd import api as A
+    from ettention_amd import synthetic as S
+
+    wl = WORKLOADS["c4"]
+    dev = torch.device("cuda:0")
+    comp = S.batch_composition(wl["members"], 1004)
+    t0 = time.time()
+    members = []
+    for i, (e, n) in enumerate(comp):
+        vg = E.VolumeGeometry.make((e, e, e))
+        base = E.ScannerGeometry.mrc_base(e, e)
+        tilts = np.linspace(-60.0, 60.0, n).astype(np.float32)
+        ph = S.ellipsoid_phantom((e, e, e), seed=4000 + i, device=dev)
+        projs = torch.zeros((n, e, e), dtype=torch.float32, device=dev)
+        scratch = torch.zeros((e, e), dtype=torch.float32, device=dev)
+        for k, t in enumerate(tilts):
+            scratch.zero_()
+            A.forward_project(vg, ph, A.rotate(base, float(t)), projs[k], scratch)
+        projs = S.add_gaussian_noise(projs, 0.02, seed=7000 + i)
+        members.append(dict(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs,
+                            algorithm=wl["algorithm"], lam=wl["lam"], projection_iteration=wl["order"],
+                            voxel_type=args.voxel_type))
+        del ph, scratch
+    torch.cuda.synchronize()
+    log("synthetic batch of %d members ready in %.1f s" % (len(members), time.time() - t0))
+    flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
+    b = A.BatchReconstruction(members, flags=flags)
+    fused = all(b.uses_fused_path(i) for i in range(len(b)))
+    for _ in range(args.warmup):
+        b.run_iterations(1)
+    b.synchronize()
+    tiles = sorted({(comp[i][0],) + b.tile(i) for i in range(len(b))}) if fused else []
+    log("warmup done (fused: %s, tiles (edge, planes, band): %s)" % (fused, tiles))
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    b.run_iterations(args.steps)
+    b.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    updates = sum(e ** 3 * n for e, n in comp)
+    gups = updates * args.steps / elapsed / 1e9
+    ms_per_step = 1e3 * elapsed / args.steps
+    log("timed %d iterations: %.2f ms/iteration, %.1f GUPS" % (args.steps, ms_per_step, gups))
+    roofline = None
+    if fused:
+        s = 4 if args.voxel_type == "float" else 2
+        kernel_ms, launches = b.time_iteration()
+        alg = sum((3 * s * e ** 3 + 20 * e * e) * n for e, n in comp)
+        achieved = alg / (kernel_ms * 1e-3) / 1e9
+        log("fused launches: %d per iteration, %.2f ms of kernel time -> %.1f GB/s algorithmic"
+            % (launches, kernel_ms, achieved))
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "fused_plane_kernel over all members (one launch per pass index)",
+                    "avg_launch_ms": round(kernel_ms / max(launches, 1), 5),
+                    "algorithmic_bytes_per_launch": int(alg / max(launches, 1)), "timed_launches": launches}
+    out = {
+        "metric": "batched SART GUPS (proj+backproj), 64 members",
+        "value": round(gups, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"float": "f32", "half": "f16-storage/f32-accumulate",
+                  "bfloat16": "bf16-storage/f32-accumulate"}[args.voxel_type],
+        "data": "synthetic (per-member seeded ellipsoid phantoms, FP projections + 2% Gaussian noise)",
+        "config": {"workload": wl["name"], "members": len(comp), "updates_per_iteration": updates,
+                   "voxel_type": args.voxel_type, "kernels": "fused-plane batch" if fused else "generic",
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

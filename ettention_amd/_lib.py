@@ -25,6 +25,7 @@ ETN_VOXEL_BFLOAT16 = 2
 ETN_ALL_SUBVOLUMES = 0xFFFFFFFF
 ETN_RECON_FLAG_GENERIC_KERNELS = 1
 ETN_RECON_FLAG_NO_GRAPH = 2
+ETN_RECON_FLAG_NO_AUTOTUNE = 4
 
 
 class EtnError(RuntimeError):
@@ -127,6 +128,17 @@ SIGNATURES = [
     ("etn_reconstruction_uses_fused_path", _i32, [_vp, C.POINTER(C.c_int)]),
     ("etn_reconstruction_time_kernels", _i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                                C.POINTER(C.c_uint32)]),
+    ("etn_reconstruction_set_tile", _i32, [_vp, C.c_int, C.c_int]),
+    ("etn_reconstruction_tile", _i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("etn_batch_create", _i32, [C.POINTER(ReconstructionDesc), _u32, _u32, _vp, C.POINTER(_vp)]),
+    ("etn_batch_destroy", _i32, [_vp]),
+    ("etn_batch_run_iterations", _i32, [_vp, _u32]),
+    ("etn_batch_synchronize", _i32, [_vp]),
+    ("etn_batch_volume", _i32, [_vp, _u32, C.POINTER(_vp), C.POINTER(C.c_size_t)]),
+    ("etn_batch_residual_norm", _i32, [_vp, _u32, C.POINTER(C.c_double)]),
+    ("etn_batch_uses_fused_path", _i32, [_vp, _u32, C.POINTER(C.c_int)]),
+    ("etn_batch_tile", _i32, [_vp, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("etn_batch_time_iteration", _i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
 ]
 
 _lib = None

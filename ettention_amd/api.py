@@ -151,6 +151,39 @@ def convert_volume(src, dst, stream=None):
 
 # ---- reconstruction ---------------------------------------------------------
 
+def _make_desc(*, volume, base_geometry, tilt_angles, projections, algorithm="sart", block_size=0, iterations=1,
+               lam=0.3, use_long_object_compensation=False, fp_samples=1, bp_samples=1,
+               projection_iteration="random", skip_projections=(), voxel_type="float", init_value=0.0,
+               x_axis_tilt=0.0, mask=None, flags=0):
+    """etn_reconstruction_desc plus the host objects it points into (keep them alive)."""
+    tilts = np.ascontiguousarray(tilt_angles, np.float32)
+    skip = np.asarray(list(skip_projections) or [0], np.uint32)
+    alg = algorithm.encode()
+    order = projection_iteration.encode() if projection_iteration else None
+    d = ReconstructionDesc()
+    d.algorithm = alg
+    d.block_size = int(block_size)
+    d.iterations = int(iterations)
+    d.lambda_ = float(lam)
+    d.use_long_object_compensation = int(bool(use_long_object_compensation))
+    d.fp_samples = int(fp_samples)
+    d.bp_samples = int(bp_samples)
+    d.projection_iteration = order
+    d.skip_projections = skip.ctypes.data_as(C.POINTER(C.c_uint32))
+    d.skip_count = len(skip_projections)
+    d.voxel_type = voxel_type_id(voxel_type)
+    d.volume = volume
+    d.init_value = float(init_value)
+    d.base_geometry = base_geometry
+    d.projection_count = len(tilts)
+    d.tilt_angles = tilts.ctypes.data_as(C.POINTER(C.c_float))
+    d.x_axis_tilt = float(x_axis_tilt)
+    d.projections = _ptr(projections)
+    d.mask = _ptr(mask)
+    d.flags = int(flags)
+    return d, (tilts, skip, alg, order, projections, mask)
+
+
 class BlockIterativeReconstructionOperator:
     """Wraps etn_reconstruction_* (BlockIterativeReconstructionOperator).
 
@@ -161,33 +194,14 @@ class BlockIterativeReconstructionOperator:
                  algorithm="sart", block_size=0, iterations=1, lam=0.3, use_long_object_compensation=False,
                  fp_samples=1, bp_samples=1, projection_iteration="random", skip_projections=(),
                  voxel_type="float", init_value=0.0, x_axis_tilt=0.0, mask=None, flags=0, stream=None):
-        self._tilts = np.ascontiguousarray(tilt_angles, np.float32)
-        self._skip = np.asarray(list(skip_projections) or [0], np.uint32)
-        self._algorithm = algorithm.encode()
-        self._order = projection_iteration.encode() if projection_iteration else None
-        self._projections = projections
-        self._mask = mask
-        d = ReconstructionDesc()
-        d.algorithm = self._algorithm
-        d.block_size = int(block_size)
-        d.iterations = int(iterations)
-        d.lambda_ = float(lam)
-        d.use_long_object_compensation = int(bool(use_long_object_compensation))
-        d.fp_samples = int(fp_samples)
-        d.bp_samples = int(bp_samples)
-        d.projection_iteration = self._order
-        d.skip_projections = self._skip.ctypes.data_as(C.POINTER(C.c_uint32))
-        d.skip_count = len(skip_projections)
-        d.voxel_type = voxel_type_id(voxel_type)
-        d.volume = volume
-        d.init_value = float(init_value)
-        d.base_geometry = base_geometry
-        d.projection_count = len(self._tilts)
-        d.tilt_angles = self._tilts.ctypes.data_as(C.POINTER(C.c_float))
-        d.x_axis_tilt = float(x_axis_tilt)
-        d.projections = _ptr(projections)
-        d.mask = _ptr(mask)
-        d.flags = int(flags)
+        d, self._keep = _make_desc(volume=volume, base_geometry=base_geometry, tilt_angles=tilt_angles,
+                                   projections=projections, algorithm=algorithm, block_size=block_size,
+                                   iterations=iterations, lam=lam,
+                                   use_long_object_compensation=use_long_object_compensation,
+                                   fp_samples=fp_samples, bp_samples=bp_samples,
+                                   projection_iteration=projection_iteration, skip_projections=skip_projections,
+                                   voxel_type=voxel_type, init_value=init_value, x_axis_tilt=x_axis_tilt,
+                                   mask=mask, flags=flags)
         self.desc = d
         self.voxel_type = d.voxel_type
         self.shape = (volume.resolution[2], volume.resolution[1], volume.resolution[0])
@@ -237,6 +251,16 @@ class BlockIterativeReconstructionOperator:
         call("etn_reconstruction_time_kernels", self._h, C.byref(k), C.byref(ms), C.byref(n))
         return k.value, ms.value, n.value
 
+    def set_tile(self, planes, band):
+        """Fused kernels' tile: volume planes per workgroup, z rows per LDS band."""
+        call("etn_reconstruction_set_tile", self._h, int(planes), int(band))
+
+    def tile(self):
+        """(planes, band) in use; runs the autotuner first if it has not run."""
+        p, b = C.c_int(), C.c_int()
+        call("etn_reconstruction_tile", self._h, C.byref(p), C.byref(b))
+        return p.value, b.value
+
     def volume_pointer(self):
         p = C.c_void_p()
         n = C.c_size_t()
@@ -264,3 +288,82 @@ class BlockIterativeReconstructionOperator:
         torch.cuda.synchronize()
         call("etn_reconstruction_set_volume", self._h, _ptr(tensor))
         self.synchronize()
+
+
+class BatchReconstruction:
+    """Wraps etn_batch_*: independent reconstructions advanced together, each
+    fused launch carrying one pass of every member.  `members` is a list of
+    keyword dicts as for BlockIterativeReconstructionOperator (only its
+    ETN_RECON_FLAG_GENERIC_KERNELS flag is honoured per member); `flags` applies
+    to the whole batch.  No reference counterpart."""
+
+    def __init__(self, members, flags=0, stream=None):
+        descs, self._keep, self.shapes, self.voxel_types = [], [], [], []
+        for m in members:
+            d, keep = _make_desc(**m)
+            descs.append(d)
+            self._keep.append(keep)
+            r = m["volume"].resolution
+            self.shapes.append((r[2], r[1], r[0]))
+            self.voxel_types.append(d.voxel_type)
+        arr = (ReconstructionDesc * len(descs))(*descs)
+        self._descs = arr
+        h = C.c_void_p()
+        call("etn_batch_create", arr, len(descs), int(flags), C.c_void_p(stream) if stream else None, C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().etn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def run_iterations(self, n):
+        call("etn_batch_run_iterations", self._h, int(n))
+
+    def synchronize(self):
+        call("etn_batch_synchronize", self._h)
+
+    def residual_norm(self, i):
+        out = C.c_double()
+        call("etn_batch_residual_norm", self._h, int(i), C.byref(out))
+        return out.value
+
+    def uses_fused_path(self, i):
+        out = C.c_int()
+        call("etn_batch_uses_fused_path", self._h, int(i), C.byref(out))
+        return bool(out.value)
+
+    def tile(self, i):
+        p, b = C.c_int(), C.c_int()
+        call("etn_batch_tile", self._h, int(i), C.byref(p), C.byref(b))
+        return p.value, b.value
+
+    def time_iteration(self):
+        """(total ms of the fused launches, launch count) over one more eager iteration."""
+        ms, n = C.c_double(), C.c_uint32()
+        call("etn_batch_time_iteration", self._h, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def get_reconstructed_volume(self, i, dtype=None):
+        """Copy of member i's volume as a device tensor [z][y][x]."""
+        torch = _torch()
+        vt = self.voxel_types[i]
+        tdt = {ETN_VOXEL_FLOAT32: torch.float32, ETN_VOXEL_FLOAT16: torch.float16,
+               ETN_VOXEL_BFLOAT16: torch.bfloat16}[vt]
+        out = torch.empty(self.shapes[i], dtype=dtype or tdt, device="cuda")
+        self.synchronize()
+        p, n = C.c_void_p(), C.c_size_t()
+        call("etn_batch_volume", self._h, int(i), C.byref(p), C.byref(n))
+        count = int(np.prod(self.shapes[i]))
+        call("etn_convert_volume", p, vt, _ptr(out), _voxel_type_of(out), count, _stream(None))
+        torch.cuda.synchronize()
+        return out

@@ -25,6 +25,7 @@ SOURCES = [
     "projection_set.cpp",
     "operators.cpp",
     "block_iterative.cpp",
+    "batch.cpp",
     "capi.cpp",
 ]
 

@@ -93,7 +93,11 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
         real_.push_back({const_cast<float*>(realProjection(i)), d.base_geometry.resolution[0], d.base_geometry.resolution[1]});
     norm_scratch_.allocate(sizeof(double) * (dev::kSsdBlocks + 1));
 
-    if ((d.flags & ETN_RECON_FLAG_GENERIC_KERNELS) == 0) fused_ = FusedPlaneEngine::create(*this, d, stream_);
+    if ((d.flags & ETN_RECON_FLAG_GENERIC_KERNELS) == 0) {
+        std::vector<bool> taken;
+        fused_ = FusedPlaneEngine::create({{this, &d}}, stream_, &taken);
+    }
+    tuned_ = (d.flags & ETN_RECON_FLAG_NO_AUTOTUNE) != 0;
 }
 
 BlockIterativeReconstructionOperator::~BlockIterativeReconstructionOperator()
@@ -130,9 +134,32 @@ void BlockIterativeReconstructionOperator::run() { runNiterations(iterations_); 
 
 void BlockIterativeReconstructionOperator::runSingleIteration() { runNiterations(1); }
 
+void BlockIterativeReconstructionOperator::ensureTuned()
+{
+    if (tuned_ || !fused_) return;
+    invalidate_graph();
+    fused_->autotune(stream_);
+    tuned_ = true;
+}
+
+void BlockIterativeReconstructionOperator::setTile(int planes, int band)
+{
+    if (!fused_) unsupported("this reconstruction does not use the fused plane kernels");
+    invalidate_graph();
+    fused_->setTile(0, planes, band, stream_);
+    tuned_ = true;
+}
+
+void BlockIterativeReconstructionOperator::tile(int* planes, int* band) const
+{
+    if (!fused_) unsupported("this reconstruction does not use the fused plane kernels");
+    fused_->tile(0, planes, band);
+}
+
 void BlockIterativeReconstructionOperator::runNiterations(uint32_t iterations)
 {
     if (iterations == 0) return;
+    ensureTuned();
     if (!use_graph_) {
         for (uint32_t i = 0; i < iterations; ++i) enqueueIteration();
         return;
@@ -249,10 +276,19 @@ void BlockIterativeReconstructionOperator::synchronize()
 double BlockIterativeReconstructionOperator::timeKernels(int* kernel, uint32_t* launches)
 {
     if (fused_) {
+        // average over the launches that carry both a back and a forward projection
+        ensureTuned();
         *kernel = 0;
-        const double ms = fused_->timeKernels(stream_, launches);
+        double total = 0.0;
+        uint32_t n = 0;
+        for (const auto& t : fused_->timeIteration(stream_))
+            if (t.steady) {
+                total += t.ms;
+                ++n;
+            }
         fused_->check(stream_);
-        return ms;
+        *launches = n;
+        return n ? total / n : 0.0;
     }
     // one eager iteration with an event pair around every FP and BP launch
     std::vector<hipEvent_t> fp_ev, bp_ev;

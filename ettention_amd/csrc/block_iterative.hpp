@@ -36,6 +36,12 @@ public:
     hipStream_t stream() const { return stream_; }
     bool usesFusedPath() const { return fused_ != nullptr; }
     void invalidate_graph();
+    // Fused path only: runs the tile autotuner once (unless disabled or a tile was set).
+    void ensureTuned();
+    void setTile(int planes, int band);
+    void tile(int* planes, int* band) const;
+    // The launches of one iteration on stream(), without graph capture.
+    void enqueueIteration();
 
     // BlockIterativeProjectionAccess (BlockIterativeProjectionAccess.cpp:11-110)
     uint32_t GetBlockCount() const { return block_count_; }
@@ -45,7 +51,6 @@ public:
     const float* realProjection(uint32_t physical) const { return projections_ + (size_t)physical * pixels_; }
 
 private:
-    void enqueueIteration();
     void handleOneIteration();
     void doForwardProjectionForBlock(uint32_t block);
     void computeResidualsForBlock(uint32_t block);
@@ -84,6 +89,7 @@ private:
     std::unique_ptr<FusedPlaneEngine> fused_;
 
     bool use_graph_;
+    bool tuned_ = true;
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

@@ -5,6 +5,7 @@
 #include <new>
 #include <string>
 
+#include "batch.hpp"
 #include "block_iterative.hpp"
 #include "etn/etn.h"
 
@@ -50,6 +51,10 @@ hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 struct etn_reconstruction {
     std::unique_ptr<etn::BlockIterativeReconstructionOperator> op;
+};
+
+struct etn_batch {
+    std::unique_ptr<etn::BatchReconstruction> batch;
 };
 
 extern "C" {
@@ -346,6 +351,107 @@ etn_status etn_reconstruction_time_kernels(etn_reconstruction* r, int* kernel, d
         need(average_ms, "average_ms");
         need(launches, "launches");
         *average_ms = r->op->timeKernels(kernel, launches);
+    });
+}
+
+etn_status etn_reconstruction_set_tile(etn_reconstruction* r, int planes, int band)
+{
+    return guarded([&] {
+        need(r, "reconstruction");
+        r->op->setTile(planes, band);
+    });
+}
+
+etn_status etn_reconstruction_tile(etn_reconstruction* r, int* planes, int* band)
+{
+    return guarded([&] {
+        need(r, "reconstruction");
+        need(planes, "planes");
+        need(band, "band");
+        r->op->ensureTuned();
+        r->op->tile(planes, band);
+    });
+}
+
+etn_status etn_batch_create(const etn_reconstruction_desc* members, uint32_t count, uint32_t flags, void* stream,
+                            etn_batch** out)
+{
+    return guarded([&] {
+        need(members, "member descriptions");
+        need(out, "output");
+        *out = nullptr;
+        auto b = std::make_unique<etn_batch>();
+        b->batch = std::make_unique<etn::BatchReconstruction>(members, count, flags, as_stream(stream));
+        *out = b.release();
+    });
+}
+
+etn_status etn_batch_destroy(etn_batch* b)
+{
+    return guarded([&] { delete b; });
+}
+
+etn_status etn_batch_run_iterations(etn_batch* b, uint32_t iterations)
+{
+    return guarded([&] {
+        need(b, "batch");
+        b->batch->runNiterations(iterations);
+    });
+}
+
+etn_status etn_batch_synchronize(etn_batch* b)
+{
+    return guarded([&] {
+        need(b, "batch");
+        b->batch->synchronize();
+    });
+}
+
+etn_status etn_batch_volume(etn_batch* b, uint32_t member, void** device_volume, size_t* bytes)
+{
+    return guarded([&] {
+        need(b, "batch");
+        etn::GpuVolume* v = b->batch->member(member).getReconstructedVolume();
+        if (device_volume) *device_volume = v->data();
+        if (bytes) *bytes = v->bytes();
+    });
+}
+
+etn_status etn_batch_residual_norm(etn_batch* b, uint32_t member, double* out)
+{
+    return guarded([&] {
+        need(b, "batch");
+        need(out, "output");
+        *out = b->batch->member(member).residualNorm();
+    });
+}
+
+etn_status etn_batch_uses_fused_path(etn_batch* b, uint32_t member, int* out)
+{
+    return guarded([&] {
+        need(b, "batch");
+        need(out, "output");
+        *out = b->batch->usesFusedPath(member) ? 1 : 0;
+    });
+}
+
+etn_status etn_batch_tile(etn_batch* b, uint32_t member, int* planes, int* band)
+{
+    return guarded([&] {
+        need(b, "batch");
+        need(planes, "planes");
+        need(band, "band");
+        b->batch->tile(member, planes, band);
+    });
+}
+
+etn_status etn_batch_time_iteration(etn_batch* b, double* fused_ms, uint32_t* launches)
+{
+    return guarded([&] {
+        need(b, "batch");
+        need(fused_ms, "fused_ms");
+        need(launches, "launches");
+        *fused_ms = b->batch->timeIteration(launches);
     });
 }
 

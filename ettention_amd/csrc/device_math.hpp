@@ -18,18 +18,44 @@ namespace dev {
 
 constexpr float kEpsilon = 1e-4f;     // ocl_src/Math.hl:4
 
+// Global-memory pointers.  A pointer read from a descriptor table in memory is
+// a generic pointer to the compiler, and accesses through it become flat
+// instructions (issued to both the memory and the LDS pipes, results in
+// VGPRs); marking it global gives global loads and scalar loads of uniform data.
+// (The host pass of the single-source compile only parses device code: no
+// address space there.)
+#ifdef __HIP_DEVICE_COMPILE__
+#define ETN_GLOBAL __attribute__((address_space(1)))
+#else
+#define ETN_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ ETN_GLOBAL T* global_ptr(T* p)
+{
+    return (ETN_GLOBAL T*)p;
+}
+
 // ---- voxel storage ---------------------------------------------------------
+// load/store take any pointer to T (generic or global)
 struct F32 {
     using T = float;
-    static __device__ __forceinline__ float load(const float* p, size_t i) { return p[i]; }
+    template <typename P>
+    static __device__ __forceinline__ float load(P p, size_t i) { return p[i]; }
     static __device__ __forceinline__ float round(float f) { return f; }
-    static __device__ __forceinline__ void store(float* p, size_t i, float f) { p[i] = f; }
+    template <typename P>
+    static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = f; }
 };
 struct F16 {   // vload_half / vstore_half (round to nearest even), HalfFloatSupport.hl:6-9
     using T = __half;
-    static __device__ __forceinline__ float load(const __half* p, size_t i) { return __half2float(p[i]); }
+    template <typename P>
+    static __device__ __forceinline__ float load(P p, size_t i)
+    {
+        const __half h = p[i];
+        return __half2float(h);
+    }
     static __device__ __forceinline__ float round(float f) { return __half2float(__float2half_rn(f)); }
-    static __device__ __forceinline__ void store(__half* p, size_t i, float f) { p[i] = __float2half_rn(f); }
+    template <typename P>
+    static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = __float2half_rn(f); }
 };
 struct BF16 {  // bfloat16, round to nearest even
     using T = uint16_t;
@@ -41,9 +67,11 @@ struct BF16 {  // bfloat16, round to nearest even
         return (uint16_t)(u >> 16);
     }
     static __device__ __forceinline__ float unpack(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-    static __device__ __forceinline__ float load(const uint16_t* p, size_t i) { return unpack(p[i]); }
+    template <typename P>
+    static __device__ __forceinline__ float load(P p, size_t i) { return unpack(p[i]); }
     static __device__ __forceinline__ float round(float f) { return unpack(pack(f)); }
-    static __device__ __forceinline__ void store(uint16_t* p, size_t i, float f) { p[i] = pack(f); }
+    template <typename P>
+    static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = pack(f); }
 };
 
 // ---- DDA ray traversal (ocl_src/DDARaytracer.hl:20-110, RayUtils.hl:7-35) ---

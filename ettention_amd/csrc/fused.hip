@@ -3,8 +3,9 @@
 //
 // Every ray then stays in one volume plane y = j, and the back projection of
 // a voxel in plane j reads detector rows that are constant across the plane.
-// One workgroup owns one plane.  It streams the plane through LDS in bands of
-// z rows, in the direction the rays travel in z, and for each band
+// One workgroup owns a group of consecutive planes (their x rows are
+// contiguous in HBM for every z).  It streams the planes through LDS in bands
+// of z rows, in the direction the rays travel in z, and for each band
 //   1. applies the back projections of the previous block (in block order,
 //      rounding to the storage type after each, exactly as the per-projection
 //      launches do) and writes the band back to HBM, then
@@ -15,9 +16,15 @@
 // and writes the volume once where the unfused schedule reads it three times
 // (FP, BP read, BP write); the arithmetic per ray and per voxel is the generic
 // kernels', operation for operation, so results are bit-identical to them.
+//
+// A launch serves many independent reconstructions at once: its workgroups are
+// (member, plane group) items, and each member contributes its current pass.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "block_iterative.hpp"
@@ -31,6 +38,10 @@ constexpr int kFusedThreads = 512;
 constexpr int kMaxFusedFp = 16;      // forward projections per pass
 constexpr int kMaxFusedBp = 8;       // back projections per pass
 constexpr int kMaxRaysPerThread = 8;
+constexpr int kMaxPlanes = 8;        // planes per workgroup
+constexpr int kLdsFloats = 16384;    // 64 KiB per workgroup
+constexpr int kRowPad = 2;           // zero taps on each side of a staged detector row
+constexpr int kMaxBandFloats = 12288;   // band voxels: the vector path prefetches 6 groups of 4 per thread
 
 struct FusedFp {
     float src[3], h[3], v[3], ray[3];
@@ -62,13 +73,27 @@ struct FusedVolume {
     SlabParams slab;             // the single subvolume (bounding box as VolumeProperties computes it)
     int nx, ny, nz;
     int nu, nv;
+    int planes;                  // consecutive y planes per workgroup (divides ny)
     int band;                    // z rows per LDS band
+    int rows_per_plane;          // detector rows whose rays lie in a plane, maximum over planes
     const int* row_lo;           // per plane: first detector row whose rays lie in it
     const int* row_hi;           // per plane: one past the last such row
     const int* dead_rows;        // detector rows whose rays all miss the volume in y
     int ndead;
     int* diag;                   // counts live rays found outside their plane (must stay 0)
 };
+
+struct FusedWork {
+    int member;                  // index into the launch's volume and pass tables
+    int plane;                   // first plane of the workgroup's group
+};
+
+// LDS floats a workgroup uses: the band, then (16-byte aligned) one 32-byte
+// record per padded detector column and plane, and a (m4 py, ty) pair per plane.
+__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int nu)
+{
+    return ((band * planes * nx + 3) & ~3) + planes * (8 * (nu + 2 * kRowPad) + 2);
+}
 
 namespace {
 
@@ -86,6 +111,7 @@ struct PlaneRay {
     int live;
     float stepx, stepz;
     int sx, sz;
+    int off;                     // LDS offset of the ray's plane inside a band row
 };
 
 // Advances a ray while it is inside the volume and inside band [zlo, zhi).  A
@@ -94,12 +120,13 @@ struct PlaneRay {
 // dropped when nx + nz <= 2049, since every step moves vx or vz monotonically by
 // one, so a ray can take at most nx + nz - 1 steps inside the volume.
 template <bool kCap>
-__device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int nx, int nz, int zlo, int zhi)
+__device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int width, int nx, int nz, int zlo,
+                                              int zhi)
 {
     if (!r.live) return;
     const unsigned h = (unsigned)(zhi - zlo);
-    const int lz = r.sz * nx;
-    int li = (r.vz - zlo) * nx + r.vx;
+    const int lz = r.sz * width;
+    int li = (r.vz - zlo) * width + r.off + r.vx;
     int vx = r.vx, vz = r.vz;
     float rx = r.rx, rz = r.rz, acc = r.acc, accl = r.accl;
     for (;;) {
@@ -144,18 +171,24 @@ template <>
 struct GroupIo<F32, true> {
     static constexpr int G = 4;
     using V = float4;
-    static __device__ __forceinline__ V load(const float* p, size_t i) { return *reinterpret_cast<const float4*>(p + i); }
-    static __device__ __forceinline__ void unpack(const V& v, float* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
-    static __device__ __forceinline__ void store(float* p, size_t i, const float* f)
+    static __device__ __forceinline__ V load(const ETN_GLOBAL float* p, size_t i)
     {
-        *reinterpret_cast<float4*>(p + i) = make_float4(f[0], f[1], f[2], f[3]);
+        return *reinterpret_cast<const ETN_GLOBAL float4*>(p + i);
+    }
+    static __device__ __forceinline__ void unpack(const V& v, float* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+    static __device__ __forceinline__ void store(ETN_GLOBAL float* p, size_t i, const float* f)
+    {
+        *reinterpret_cast<ETN_GLOBAL float4*>(p + i) = make_float4(f[0], f[1], f[2], f[3]);
     }
 };
 template <typename S16>
 struct GroupIo16 {   // 4 x 16-bit voxels as one 8-byte access
     static constexpr int G = 4;
     using V = uint2;
-    static __device__ __forceinline__ V load(const typename S16::T* p, size_t i) { return *reinterpret_cast<const uint2*>(p + i); }
+    static __device__ __forceinline__ V load(const ETN_GLOBAL typename S16::T* p, size_t i)
+    {
+        return *reinterpret_cast<const ETN_GLOBAL uint2*>(p + i);
+    }
     static __device__ __forceinline__ void unpack(const V& v, float* o)
     {
         o[0] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 0);
@@ -163,12 +196,12 @@ struct GroupIo16 {   // 4 x 16-bit voxels as one 8-byte access
         o[2] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 2);
         o[3] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 3);
     }
-    static __device__ __forceinline__ void store(typename S16::T* p, size_t i, const float* f)
+    static __device__ __forceinline__ void store(ETN_GLOBAL typename S16::T* p, size_t i, const float* f)
     {
         V v;
         typename S16::T* t = reinterpret_cast<typename S16::T*>(&v);
         for (int k = 0; k < 4; ++k) S16::store(t, k, f[k]);
-        *reinterpret_cast<uint2*>(p + i) = v;
+        *reinterpret_cast<ETN_GLOBAL uint2*>(p + i) = v;
     }
 };
 template <>
@@ -179,45 +212,71 @@ template <typename S>
 struct GroupIo<S, false> {
     static constexpr int G = 1;
     using V = typename S::T;
-    static __device__ __forceinline__ V load(const typename S::T* p, size_t i) { return p[i]; }
+    static __device__ __forceinline__ V load(const ETN_GLOBAL typename S::T* p, size_t i) { return p[i]; }
     static __device__ __forceinline__ void unpack(const V& v, float* o) { o[0] = S::load(&v, 0); }
-    static __device__ __forceinline__ void store(typename S::T* p, size_t i, const float* f) { S::store(p, i, f[0]); }
+    static __device__ __forceinline__ void store(ETN_GLOBAL typename S::T* p, size_t i, const float* f)
+    {
+        S::store(p, i, f[0]);
+    }
 };
 
-constexpr int kLdsFloats = 16384;   // 64 KiB per workgroup
-constexpr int kRowPad = 2;          // zero taps on each side of a staged detector row
-
-// LinearInterpolation (ImageHandler.hl:20-29) on a staged row with kRowPad zeros on
+// LinearInterpolation (ImageHandler.hl:20-29) on staged rows with kRowPad zeros on
 // each side, stored as (v[i], v[i+1] - v[i]) pairs: clamping the base index to
 // [-2, n] reads only zeros wherever the reference's range checks return 0, and
 // the stored difference is the subtraction mix() performs, so the value is the same.
+// Column i of the two residual rows and the two length rows forms one 32-byte
+// record {r0, dr0, r1, dr1}, {l0, dl0, l1, dl1}: one address, two 16-byte reads.
 __device__ __forceinline__ int padded_index(int i0, int n) { return min(max(i0, -kRowPad), n) + kRowPad; }
-__device__ __forceinline__ float lerp_pair(float2 p, float t) { return p.x + p.y * t; }
+__device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a + d * t; }
 
+// One workgroup per (member, plane group) work item.  `passes[m]` is member m's
+// pass of this launch; members without one get no work items.
 template <typename S, bool kVec, bool kCap, int RPT>
-__global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume fv, const FusedPass* __restrict__ pass)
+__global__ void __launch_bounds__(kFusedThreads)
+    fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
+                       const FusedPass* const* __restrict__ passes)
 {
     using IO = GroupIo<S, kVec>;
     constexpr int G = IO::G;
-    constexpr int kGroups = kLdsFloats / (G * kFusedThreads);   // band groups per thread (upper bound)
+    constexpr int kGroups = kMaxBandFloats / (G * kFusedThreads);   // band groups per thread (upper bound)
     extern __shared__ float lds[];
-    const int j = blockIdx.x;
+    const FusedWork item = work[blockIdx.x];
+    const ETN_GLOBAL FusedVolume& fv = global_ptr(vols)[item.member];
+    const ETN_GLOBAL FusedPass* pass = global_ptr(global_ptr(passes)[item.member]);
+    const int j = item.plane;                           // planes j .. j + np - 1
     const int tid = threadIdx.x;
-    const int nx = fv.nx, nz = fv.nz, nu = fv.nu;
+    const int nx = fv.nx, nz = fv.nz, nu = fv.nu, np = fv.planes;
+    const int width = np * nx;                          // one band row: the x rows of the np planes
     const int row_len = nu + 2 * kRowPad;
-    float* band = lds;                                  // band x nx
-    // 4 padded detector rows of (value, difference), 8-byte aligned after the band
-    float2* rows = reinterpret_cast<float2*>(lds + ((fv.band * nx + 1) & ~1));
-    typename S::T* vol = static_cast<typename S::T*>(fv.data);
+    float* band = lds;                                  // band x width
+    // per plane row_len column records, 16-byte aligned after the band, then
+    // per plane (m4 py, ty) of the current back projection
+    float4* recs = reinterpret_cast<float4*>(lds + ((fv.band * width + 3) & ~3));
+    float2* pterm = reinterpret_cast<float2*>(recs + np * 2 * row_len);
+    ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
     const size_t plane_stride = (size_t)nx * fv.ny;
     const size_t plane_base = (size_t)j * nx;
-    const int gpr = nx / G;                             // groups per band row
+    const int gpr = width / G;                          // groups per band row
 
     const int nfp = pass->nfp, nbp = pass->nbp;
-    const int row_lo = fv.row_lo[j], nrows = fv.row_hi[j] - row_lo;
-    const int rays_per_proj = nrows * nu;
+    const int R = fv.rows_per_plane;
+    const int rays_per_proj = np * R * nu;
     const int nrays = nfp * rays_per_proj;
-    const SlabParams& s = fv.slab;
+    const SlabParams s = fv.slab;
+    const ETN_GLOBAL int* row_lo = global_ptr(fv.row_lo);
+    const ETN_GLOBAL int* row_hi = global_ptr(fv.row_hi);
+
+    // ray slot id = ((q * np + p) * R + r) * nu + u: projection q, plane j + p,
+    // its r-th detector row w, column u; slots past a plane's rows stay idle
+    auto decode = [&](int id, int& q, int& p, int& w, int& u) {
+        q = id / rays_per_proj;
+        const int rem = id - q * rays_per_proj;
+        const int pr = rem / nu;
+        u = rem - pr * nu;
+        p = pr / R;
+        w = row_lo[j + p] + (pr - p * R);
+        return w < row_hi[j + p];
+    };
 
     // ---- ray set-up: dda_begin exactly as the generic FP kernel ----
     PlaneRay ray[RPT];
@@ -227,12 +286,10 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
         ray[k].live = 0;
         ray[k].acc = 0.0f;
         ray[k].accl = 0.0f;
-        if (id < nrays) {
-            const int q = id / rays_per_proj;
-            const int rem = id - q * rays_per_proj;
-            const int w = row_lo + rem / nu;
-            const int u = rem - (rem / nu) * nu;
-            const FusedFp& f = pass->fp[q];
+        ray[k].off = 0;
+        int q, p, w, u;
+        if (id < nrays && decode(id, q, p, w, u)) {
+            const ETN_GLOBAL FusedFp& f = pass->fp[q];
             RayParams g;
             for (int a = 0; a < 3; ++a) {
                 g.source[a] = f.src[a];
@@ -240,12 +297,12 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
                 g.v[a] = f.v[a];
                 g.ray[a] = f.ray[a];
             }
-            const DdaConst c = dda_const(f.ray, s.vs);
+            const DdaConst c = dda_const(g.ray, s.vs);
             float src[3];
             fp_ray_source(g, (uint32_t)u, (uint32_t)w, 0, 0, 1.0f, 0.5f, src);
             DdaRay r;
             dda_begin(s, c, src, r);
-            if (r.live && r.vox[1] != j) {
+            if (r.live && r.vox[1] != j + p) {
                 atomicAdd(fv.diag, 1);
                 r.live = 0;
             }
@@ -259,35 +316,53 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
             ray[k].stepz = c.step[2];
             ray[k].sx = c.sign[0];
             ray[k].sz = c.sign[2];
+            ray[k].off = p * nx;
         }
     }
 
     const int nbands = (nz + fv.band - 1) / fv.band;
-    const float py = s.bmin[1] + ((float)j + 0.5f) * s.vs[1];
 
-    // detector rows of back projection q: the v coordinate is constant over the
-    // plane (checked on the host), so rows j0, j0+1 and the weight ty are too
-    int j0 = 0;
-    float ty = 0.0f;
-    auto stage_rows = [&](int q) {
-        const FusedBp& bp = pass->bp[q];
+    // v coordinate of plane j + p under back projection bp, constant over the
+    // plane (checked on the host): returns detector row j0, sets the weight ty
+    // and the plane's voxel-centre y
+    auto plane_v = [&](const ETN_GLOBAL FusedBp& bp, int p, float& py, float& ty) {
+        py = s.bmin[1] + ((float)(j + p) + 0.5f) * s.vs[1];
         float vp = bp.m[1] * 0.0f + bp.m[5] * (py + bp.sample.y);
         vp = vp + bp.m[9] * 0.0f;
         vp = vp + bp.m[13] * 1.0f;
         float base_row;
         ty = cl_fract(vp - 0.5f, &base_row);
-        j0 = (int)base_row;
-        for (int e = tid; e < 2 * row_len; e += kFusedThreads) {
-            const int rr = e / row_len, k = e - rr * row_len, u = k - kRowPad;
-            const int row = j0 + rr;
-            const bool rok = row >= 0 && row < fv.nv;
-            const bool ok0 = rok && u >= 0 && u < nu, ok1 = rok && u + 1 >= 0 && u + 1 < nu;
-            const float r0 = ok0 ? bp.residual[(size_t)row * nu + u] : 0.0f;
-            const float r1 = ok1 ? bp.residual[(size_t)row * nu + u + 1] : 0.0f;
-            const float l0 = ok0 ? bp.length[(size_t)row * nu + u] : 0.0f;
-            const float l1 = ok1 ? bp.length[(size_t)row * nu + u + 1] : 0.0f;
-            rows[rr * row_len + k] = make_float2(r0, r1 - r0);
-            rows[(2 + rr) * row_len + k] = make_float2(l0, l1 - l0);
+        return (int)base_row;
+    };
+    auto stage_rows = [&](int q) {
+        const ETN_GLOBAL FusedBp& bp = pass->bp[q];
+        const ETN_GLOBAL float* res = global_ptr(bp.residual);
+        const ETN_GLOBAL float* len = global_ptr(bp.length);
+        for (int e = tid; e < np * row_len; e += kFusedThreads) {
+            const int p = e / row_len, k = e - p * row_len, u = k - kRowPad;
+            float py, ty;
+            const int row = plane_v(bp, p, py, ty);
+            float v[8];   // (value, difference) of residual row, row + 1, length row, row + 1
+            for (int rr = 0; rr < 2; ++rr) {
+                const bool rok = row + rr >= 0 && row + rr < fv.nv;
+                const bool ok0 = rok && u >= 0 && u < nu, ok1 = rok && u + 1 >= 0 && u + 1 < nu;
+                const size_t at = (size_t)(row + rr) * nu + u;
+                const float r0 = ok0 ? res[at] : 0.0f;
+                const float r1 = ok1 ? res[at + 1] : 0.0f;
+                const float l0 = ok0 ? len[at] : 0.0f;
+                const float l1 = ok1 ? len[at + 1] : 0.0f;
+                v[2 * rr] = r0;
+                v[2 * rr + 1] = r1 - r0;
+                v[4 + 2 * rr] = l0;
+                v[4 + 2 * rr + 1] = l1 - l0;
+            }
+            recs[2 * e] = make_float4(v[0], v[1], v[2], v[3]);
+            recs[2 * e + 1] = make_float4(v[4], v[5], v[6], v[7]);
+        }
+        if (tid < np) {
+            float py, ty;
+            plane_v(bp, tid, py, ty);
+            pterm[tid] = make_float2(bp.m[4] * (py + bp.sample.y), ty);
         }
     };
 
@@ -332,7 +407,7 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
                     IO::unpack(pre[k], v);
                     const int zz = gi / gpr, xg = gi - zz * gpr;
 #pragma unroll
-                    for (int i = 0; i < G; ++i) band[zz * nx + xg * G + i] = v[i];
+                    for (int i = 0; i < G; ++i) band[zz * width + xg * G + i] = v[i];
                 }
             }
         } else {
@@ -341,7 +416,7 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
                 float v[G];
                 IO::unpack(IO::load(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G), v);
 #pragma unroll
-                for (int i = 0; i < G; ++i) band[zz * nx + xg * G + i] = v[i];
+                for (int i = 0; i < G; ++i) band[zz * width + xg * G + i] = v[i];
             }
         }
         __syncthreads();
@@ -358,15 +433,10 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
                 stage_rows(q);
                 __syncthreads();
             }
-            const FusedBp& bp = pass->bp[q];
-            const float2* r0 = rows;
-            const float2* r1 = rows + row_len;
-            const float2* l0 = rows + 2 * row_len;
-            const float2* l1 = rows + 3 * row_len;
+            const ETN_GLOBAL FusedBp& bp = pass->bp[q];
             const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda, chord = bp.sample.w;
-            const float a4 = bp.m[4] * (py + bp.sample.y);
             // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
-            auto x_term = [&](int x) {
+            auto x_term = [&](int x, float a4) {
                 const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + bp.sample.x;
                 return m0 * px + a4;
             };
@@ -374,55 +444,70 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
                 const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + bp.sample.z;
                 return m8 * pz;
             };
-            // ParallelBeamsBackProjection.cl:26-52 for one sample
-            auto update = [&](float xt, float zt, float value) {
+            // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
+            auto update = [&](const float4* pr, float ty, float xt, float zt, float value) {
                 float u = xt + zt;
                 u = u + m12 * 1.0f;
                 float bu;
                 const float tx = cl_fract(u - 0.5f, &bu);
                 const int i = padded_index((int)bu, nu);
-                const float L0 = lerp_pair(l0[i], tx), L1 = lerp_pair(l1[i], tx);
+                const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
+                const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
                 const float L = L0 + (L1 - L0) * ty;
-                const float R0 = lerp_pair(r0[i], tx), R1 = lerp_pair(r1[i], tx);
-                const float R = R0 + (R1 - R0) * ty;
+                const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
+                const float Rv = R0 + (R1 - R0) * ty;
                 const float wgt = chord / L;
-                const float contrib = R * lam * wgt;
+                const float contrib = Rv * lam * wgt;
                 const float sum = 0.0f + contrib;             // sum over the one sample
                 value = (L > kEpsilon) ? value + sum : value; // += 1 * sum / 1 when used
                 return S::round(value);
             };
             const int nrow = z1 - z0;
             if (kFusedThreads % gpr == 0) {
-                // every thread keeps the same x group in every band row
-                const int x0 = (tid % gpr) * G, dz = kFusedThreads / gpr;
+                // every thread keeps the same x group (and plane) in every band row
+                const int xw = (tid % gpr) * G, dz = kFusedThreads / gpr;
+                int p = 0, x0 = xw;
+                if (np > 1) {
+                    p = xw / nx;
+                    x0 = xw - p * nx;
+                }
+                const float2 pt = pterm[p];
+                const float4* pr = recs + p * 2 * row_len;
                 float xt[G];
 #pragma unroll
-                for (int i = 0; i < G; ++i) xt[i] = x_term(x0 + i);
+                for (int i = 0; i < G; ++i) xt[i] = x_term(x0 + i, pt.x);
 #pragma unroll 1
                 for (int zz = tid / gpr; zz < nrow; zz += dz) {
                     const float zt = z_term(zz);
                     float out[G];
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
-                        out[i] = update(xt[i], zt, band[zz * nx + x0 + i]);
-                        band[zz * nx + x0 + i] = out[i];
+                        out[i] = update(pr, pt.y, xt[i], zt, band[zz * width + xw + i]);
+                        band[zz * width + xw + i] = out[i];
                     }
                     if (q == nbp - 1)
-                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)x0, out);
+                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
                 }
             } else {
 #pragma unroll 1
                 for (int gi = tid; gi < ng; gi += kFusedThreads) {
-                    const int zz = gi / gpr, x0 = (gi - zz * gpr) * G;
+                    const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
+                    int p = 0, x0 = xw;
+                    if (np > 1) {
+                        p = xw / nx;
+                        x0 = xw - p * nx;
+                    }
+                    const float2 pt = pterm[p];
+                    const float4* pr = recs + p * 2 * row_len;
                     const float zt = z_term(zz);
                     float out[G];
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
-                        out[i] = update(x_term(x0 + i), zt, band[zz * nx + x0 + i]);
-                        band[zz * nx + x0 + i] = out[i];
+                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[zz * width + xw + i]);
+                        band[zz * width + xw + i] = out[i];
                     }
                     if (q == nbp - 1)
-                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)x0, out);
+                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
                 }
             }
             if (nbp > 1) __syncthreads();
@@ -434,7 +519,7 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 const int id = k * kFusedThreads + tid;
-                if (id < nrays) plane_advance<kCap>(ray[k], band, nx, nz, z0, z1);
+                if (id < nrays) plane_advance<kCap>(ray[k], band, width, nx, nz, z0, z1);
             }
         }
         __syncthreads();
@@ -444,78 +529,83 @@ __global__ void __launch_bounds__(kFusedThreads) fused_plane_kernel(FusedVolume 
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int id = k * kFusedThreads + tid;
-        if (id < nrays) {
-            const int q = id / rays_per_proj;
-            const int rem = id - q * rays_per_proj;
-            const int w = row_lo + rem / nu;
-            const int u = rem - (rem / nu) * nu;
-            const FusedFp& f = pass->fp[q];
+        int q, p, w, u;
+        if (id < nrays && decode(id, q, p, w, u)) {
+            const ETN_GLOBAL FusedFp& f = pass->fp[q];
             float acc = 0.0f, accl = 0.0f;
             acc += ray[k].acc;
             accl += ray[k].accl;
-            float p = 0.0f + acc / 1.0f;
+            float pv = 0.0f + acc / 1.0f;
             float l = 0.0f + accl / 1.0f;
             if (f.use_loc) {
-                p = l > kEpsilon ? p * f.loc_c / l : 0.0f;
+                pv = l > kEpsilon ? pv * f.loc_c / l : 0.0f;
                 l = f.loc_c;
             }
             const size_t i = (size_t)w * nu + u;
-            f.residual[i] = f.measured[i] - p;
-            f.length[i] = l;
+            global_ptr(f.residual)[i] = global_ptr(f.measured)[i] - pv;
+            global_ptr(f.length)[i] = l;
         }
     }
 
     // ---- rows whose rays miss the volume: zero line integral and length ----
-    for (int d = j; d < fv.ndead; d += fv.ny) {
-        const int w = fv.dead_rows[d];
-        for (int q = 0; q < nfp; ++q) {
-            const FusedFp& f = pass->fp[q];
-            for (int u = tid; u < nu; u += kFusedThreads) {
-                float p = 0.0f + 0.0f / 1.0f;
-                float l = 0.0f + 0.0f / 1.0f;
-                if (f.use_loc) {
-                    p = l > kEpsilon ? p * f.loc_c / l : 0.0f;
-                    l = f.loc_c;
+    for (int p = 0; p < np; ++p)
+        for (int d = j + p; d < fv.ndead; d += fv.ny) {
+            const int w = global_ptr(fv.dead_rows)[d];
+            for (int q = 0; q < nfp; ++q) {
+                const ETN_GLOBAL FusedFp& f = pass->fp[q];
+                for (int u = tid; u < nu; u += kFusedThreads) {
+                    float pv = 0.0f + 0.0f / 1.0f;
+                    float l = 0.0f + 0.0f / 1.0f;
+                    if (f.use_loc) {
+                        pv = l > kEpsilon ? pv * f.loc_c / l : 0.0f;
+                        l = f.loc_c;
+                    }
+                    const size_t i = (size_t)w * nu + u;
+                    global_ptr(f.residual)[i] = global_ptr(f.measured)[i] - pv;
+                    global_ptr(f.length)[i] = l;
                 }
-                const size_t i = (size_t)w * nu + u;
-                f.residual[i] = f.measured[i] - p;
-                f.length[i] = l;
             }
         }
-    }
 }
 
+struct FusedLaunch {
+    int rpt;
+    int groups;
+    const FusedVolume* vols;
+    const FusedWork* work;
+    const FusedPass* const* passes;
+    size_t lds;
+};
+
 template <typename S, bool kVec, bool kCap>
-void launch_fused_cap(int rpt, const FusedVolume& fv, const FusedPass* pass, size_t lds, hipStream_t stream)
+void launch_fused_cap(const FusedLaunch& l, hipStream_t stream)
 {
-    const dim3 grid(fv.ny), block(kFusedThreads);
-    switch (rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, 1><<<grid, block, lds, stream>>>(fv, pass); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, 2><<<grid, block, lds, stream>>>(fv, pass); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, 4><<<grid, block, lds, stream>>>(fv, pass); break;
-        default: fused_plane_kernel<S, kVec, kCap, 8><<<grid, block, lds, stream>>>(fv, pass); break;
+    const dim3 grid(l.groups), block(kFusedThreads);
+    switch (l.rpt) {
+        case 1: fused_plane_kernel<S, kVec, kCap, 1><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, 2><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, 4><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, 8><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
     }
 }
 
 template <typename S>
-void launch_fused_typed(int rpt, const FusedVolume& fv, const FusedPass* pass, size_t lds, hipStream_t stream)
+void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
-    const bool vec = fv.nx % 4 == 0;
-    const bool cap = fv.nx + fv.nz > 2049;
-    if (vec && !cap) launch_fused_cap<S, true, false>(rpt, fv, pass, lds, stream);
-    else if (vec) launch_fused_cap<S, true, true>(rpt, fv, pass, lds, stream);
-    else if (!cap) launch_fused_cap<S, false, false>(rpt, fv, pass, lds, stream);
-    else launch_fused_cap<S, false, true>(rpt, fv, pass, lds, stream);
+    if (vec && !cap) launch_fused_cap<S, true, false>(l, stream);
+    else if (vec) launch_fused_cap<S, true, true>(l, stream);
+    else if (!cap) launch_fused_cap<S, false, false>(l, stream);
+    else launch_fused_cap<S, false, true>(l, stream);
 }
 
 }  // namespace
 
-void launch_fused(etn_voxel_type vt, int rpt, const FusedVolume& fv, const FusedPass* pass, size_t lds,
-                  hipStream_t stream)
+// vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049
+void launch_fused(etn_voxel_type vt, bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
-    if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32>(rpt, fv, pass, lds, stream);
-    else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16>(rpt, fv, pass, lds, stream);
-    else launch_fused_typed<BF16>(rpt, fv, pass, lds, stream);
+    if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32>(vec, cap, l, stream);
+    else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16>(vec, cap, l, stream);
+    else launch_fused_typed<BF16>(vec, cap, l, stream);
     hip_check(hipGetLastError(), "fused plane kernel launch");
 }
 
@@ -526,30 +616,27 @@ void launch_fused(etn_voxel_type vt, int rpt, const FusedVolume& fv, const Fused
 // ---------------------------------------------------------------------------
 namespace {
 
-class FusedPlaneEngineImpl final : public FusedPlaneEngine {
-public:
-    FusedPlaneEngineImpl(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d,
-                         std::vector<int> row_lo, std::vector<int> row_hi, std::vector<int> dead_rows,
-                         int rows_per_plane, hipStream_t stream);
-    void enqueueIteration(hipStream_t stream) override;
-    void check(hipStream_t stream) override;
-    double timeKernels(hipStream_t stream, uint32_t* launches) override;
+struct Tile {
+    int planes = 1;
+    int band = 1;
+};
 
-private:
-    void build_passes();
-    BlockIterativeReconstructionOperator& op_;
-    etn_voxel_type vt_;
-    int rows_per_plane_;
-    int nu_, nv_;
-    size_t pixels_;
-    float lambda_;
-    bool use_loc_;
-    int max_fp_;
-    dev::FusedVolume fv_;
-    size_t lds_bytes_;
-    DeviceBuffer rows_, diag_, images_, passes_dev_;
-    std::vector<dev::FusedPass> passes_;
-    std::vector<int> rpt_;
+// One reconstruction driven by the engine: its plane/row tables, its
+// residual images and the passes of one iteration under its current tile.
+struct MemberPlan {
+    BlockIterativeReconstructionOperator* op = nullptr;
+    int nx = 0, ny = 0, nz = 0, nu = 0, nv = 0;
+    size_t pixels = 0;
+    float lambda = 0.0f;
+    bool use_loc = false;
+    int rows_per_plane = 0;
+    int ndead = 0;
+    uint32_t bmax = 0;                 // largest block
+    DeviceBuffer rows;                 // row_lo[ny] | row_hi[ny] | dead rows
+    DeviceBuffer images;               // residual + length per block slot, two parities
+    Tile tile;
+    std::vector<dev::FusedPass> passes;
+    int offset = 0;                    // launch that carries the member's first pass
 };
 
 int select_rpt(int rays)
@@ -559,18 +646,58 @@ int select_rpt(int rays)
     return rpt;
 }
 
-}  // namespace
+// Largest band for `planes` planes per workgroup, 0 when the tile does not fit.
+int max_band(const MemberPlan& m, int planes)
+{
+    if (planes < 1 || planes > dev::kMaxPlanes || (planes & (planes - 1)) || m.ny % planes) return 0;
+    if (planes * m.rows_per_plane * m.nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return 0;
+    int band = std::min(m.nz, (dev::kLdsFloats - 3 - planes * (8 * (m.nu + 2 * dev::kRowPad) + 2)) / (planes * m.nx));
+    band = std::min(band, dev::kMaxBandFloats / (planes * m.nx));
+    return std::max(band, 0);
+}
 
-std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconstructionOperator& op,
-                                                           const etn_reconstruction_desc& d, hipStream_t stream)
+bool tile_fits(const MemberPlan& m, const Tile& t)
+{
+    return t.band >= 1 && t.band <= max_band(m, t.planes) &&
+           dev::fused_lds_floats(t.planes, t.band, m.nx, m.nu) <= dev::kLdsFloats;
+}
+
+// Without timing: the fewest planes whose rays fill a workgroup, the tallest band.
+Tile default_tile(const MemberPlan& m)
+{
+    Tile best{1, max_band(m, 1)};
+    for (int p = 1; p <= dev::kMaxPlanes; p *= 2) {
+        const int b = max_band(m, p);
+        if (b < 1) continue;
+        best = {p, b};
+        if (p * m.rows_per_plane * m.nu >= dev::kFusedThreads) break;
+    }
+    return best;
+}
+
+std::vector<Tile> candidate_tiles(const MemberPlan& m)
+{
+    std::vector<Tile> out;
+    for (int p = 1; p <= dev::kMaxPlanes; p *= 2) {
+        const int b = max_band(m, p);
+        if (b < 1) continue;
+        out.push_back({p, b});
+        if (b >= 2) out.push_back({p, (b + 1) / 2});
+    }
+    return out;
+}
+
+// Eligibility of one reconstruction and its plane/row tables; false when it
+// has no fused form.
+bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d, MemberPlan& out)
 {
     const VolumeProperties& vp = op.getReconstructedVolume()->properties();
-    if (vp.subvolume_count() != 1) return nullptr;
-    if ((d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1) return nullptr;
-    if (d.mask) return nullptr;
+    if (vp.subvolume_count() != 1) return false;
+    if ((d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1) return false;
+    if (d.mask) return false;
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
     const int nu = (int)d.base_geometry.resolution[0], nv = (int)d.base_geometry.resolution[1];
-    if (8 * (nu + 2 * dev::kRowPad) + nx + 1 > dev::kLdsFloats) return nullptr;   // a z row and the detector rows fit in LDS
+    if (dev::fused_lds_floats(1, 1, nx, nu) > dev::kLdsFloats) return false;   // a z row and the detector rows fit
 
     // every projection: rays in y planes, constant detector row per plane
     std::vector<int> plane_of_row;
@@ -578,15 +705,15 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
         for (uint32_t p = 0; p < op.getBlockSize(b); ++p) {
             const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
             const Vec3f r = g.ray_direction();
-            if (r.y != 0.0f || g.horizontal_pitch().y != 0.0f) return nullptr;
+            if (r.y != 0.0f || g.horizontal_pitch().y != 0.0f) return false;
             // nonzero x/z components must have finite reciprocals (the plane
             // DDA relies on every res value staying at or below FLT_MAX)
             for (float c : {r.x, r.z})
-                if (c != 0.0f && std::fabs(c) < 1e-30f) return nullptr;
+                if (c != 0.0f && std::fabs(c) < 1e-30f) return false;
             const Matrix4x4 m = g.projection_matrix();
             const float* mv = m.data();
             if (mv[1] != 0.0f || mv[9] != 0.0f || mv[3] != 0.0f || mv[7] != 0.0f || mv[11] != 0.0f || mv[15] != 1.0f)
-                return nullptr;
+                return false;
             // plane of each detector row, with the FP's own arithmetic (samples = 1);
             // -1 for rows whose sources lie outside the box in y by a margin that
             // makes every ray's y slab interval empty (t_out <= -2^-20 * FLT_MAX)
@@ -603,7 +730,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
                     map[w] = -1;
                     continue;
                 }
-                if (y < sub.bmin.y || y > sub.bmax.y) return nullptr;   // too close to call: generic kernels
+                if (y < sub.bmin.y || y > sub.bmax.y) return false;   // too close to call: generic kernels
                 y = std::fmin(std::fmax(y, sub.bmin.y), sub.bmax.y);
                 int jj = (int)((y - sub.bmin.y) * ivs);
                 jj = std::max(jj, 0);
@@ -611,7 +738,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
                 map[w] = jj;
             }
             if (plane_of_row.empty()) plane_of_row = map;
-            else if (plane_of_row != map) return nullptr;
+            else if (plane_of_row != map) return false;
         }
     std::vector<int> lo(ny, 0), hi(ny, 0), dead;
     std::vector<int> seen(ny, 0);
@@ -623,7 +750,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
             continue;
         }
         if (jj != prev) {
-            if (seen[jj]) return nullptr;        // rows of a plane must be contiguous
+            if (seen[jj]) return false;        // rows of a plane must be contiguous
             seen[jj] = 1;
             lo[jj] = w;
             prev = jj;
@@ -632,64 +759,44 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(BlockIterativeReconst
     }
     int rows_per_plane = 0;
     for (int jj = 0; jj < ny; ++jj) rows_per_plane = std::max(rows_per_plane, hi[jj] - lo[jj]);
-    if (rows_per_plane * nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return nullptr;
-    (void)nz;
-    return std::make_unique<FusedPlaneEngineImpl>(op, d, lo, hi, dead, rows_per_plane, stream);
+    if (rows_per_plane < 1 || rows_per_plane * nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return false;
+
+    out.op = &op;
+    out.nx = nx;
+    out.ny = ny;
+    out.nz = nz;
+    out.nu = nu;
+    out.nv = nv;
+    out.pixels = (size_t)nu * nv;
+    out.lambda = d.lambda;
+    out.use_loc = d.use_long_object_compensation != 0;
+    out.rows_per_plane = rows_per_plane;
+    if (max_band(out, 1) < 1) return false;   // not even one z row of one plane fits
+    out.ndead = (int)dead.size();
+    std::vector<int> table(lo);
+    table.insert(table.end(), hi.begin(), hi.end());
+    table.insert(table.end(), dead.begin(), dead.end());
+    out.rows.allocate(sizeof(int) * table.size());
+    ETN_HIP(hipMemcpy(out.rows.get(), table.data(), sizeof(int) * table.size(), hipMemcpyHostToDevice));
+    out.bmax = 0;
+    for (uint32_t b = 0; b < op.GetBlockCount(); ++b) out.bmax = std::max(out.bmax, op.getBlockSize(b));
+    out.images.allocate(sizeof(float) * out.pixels * 2 * 2 * out.bmax);
+    out.tile = default_tile(out);
+    return true;
 }
 
-FusedPlaneEngineImpl::FusedPlaneEngineImpl(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d,
-                                           std::vector<int> row_lo, std::vector<int> row_hi,
-                                           std::vector<int> dead_rows, int rows_per_plane, hipStream_t stream)
-    : op_(op), vt_(d.voxel_type), rows_per_plane_(rows_per_plane), nu_((int)d.base_geometry.resolution[0]),
-      nv_((int)d.base_geometry.resolution[1]), lambda_(d.lambda), use_loc_(d.use_long_object_compensation != 0)
+// The passes of one iteration: pass i back-projects what is left of the
+// previous block and forward-projects a group of the current block.
+void build_passes(MemberPlan& mb)
 {
-    pixels_ = (size_t)nu_ * nv_;
-    GpuVolume* vol = op.getReconstructedVolume();
-    const VolumeProperties& vp = vol->properties();
-    const int ny = (int)vp.ny();
-    std::vector<int> table(row_lo);
-    table.insert(table.end(), row_hi.begin(), row_hi.end());
-    table.insert(table.end(), dead_rows.begin(), dead_rows.end());
-    rows_.allocate(sizeof(int) * table.size());
-    ETN_HIP(hipMemcpy(rows_.get(), table.data(), sizeof(int) * table.size(), hipMemcpyHostToDevice));
-    diag_.allocate(sizeof(int));
-    ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
-
-    fv_.data = vol->data();
-    fv_.slab = make_slab_params(vp, 0);
-    fv_.nx = (int)vp.nx();
-    fv_.ny = ny;
-    fv_.nz = (int)vp.nz();
-    fv_.nu = nu_;
-    fv_.nv = nv_;
-    fv_.row_lo = rows_.as<int>();
-    fv_.row_hi = rows_.as<int>() + ny;
-    fv_.dead_rows = rows_.as<int>() + 2 * ny;
-    fv_.ndead = (int)dead_rows.size();
-    fv_.diag = diag_.as<int>();
-    // 64 KiB of LDS per workgroup: 4 padded detector rows, the rest a band of z rows
-    const int row_floats = 8 * (nu_ + 2 * dev::kRowPad);   // 4 rows of (value, difference)
-    fv_.band = std::max(1, std::min(fv_.nz, (dev::kLdsFloats - row_floats - 1) / fv_.nx));
-    lds_bytes_ = sizeof(float) * ((((size_t)fv_.band * fv_.nx + 1) & ~(size_t)1) + (size_t)row_floats);
-    max_fp_ = std::max(1, std::min(dev::kMaxFusedFp,
-                                   dev::kFusedThreads * dev::kMaxRaysPerThread / std::max(1, rows_per_plane_ * nu_)));
-    build_passes();
-}
-
-void FusedPlaneEngineImpl::build_passes()
-{
-    passes_.clear();
-    rpt_.clear();
-    const uint32_t nb = op_.GetBlockCount();
-    uint32_t bmax = 0;
-    for (uint32_t b = 0; b < nb; ++b) bmax = std::max(bmax, op_.getBlockSize(b));
-    // residual + length images per block slot, ping-pong between consecutive blocks
-    images_.allocate(sizeof(float) * pixels_ * 2 * 2 * bmax);
+    BlockIterativeReconstructionOperator& op = *mb.op;
+    mb.passes.clear();
+    const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, dev::kFusedThreads * dev::kMaxRaysPerThread /
+                                                                  (mb.tile.planes * mb.rows_per_plane * mb.nu)));
     auto image = [&](uint32_t parity, uint32_t slot, int which) {
-        return images_.as<float>() + pixels_ * ((((size_t)parity * bmax + slot) * 2) + which);
+        return mb.images.as<float>() + mb.pixels * ((((size_t)parity * mb.bmax + slot) * 2) + which);
     };
-    const VolumeProperties& vp = op_.getReconstructedVolume()->properties();
-
+    const VolumeProperties& vp = op.getReconstructedVolume()->properties();
     std::vector<dev::FusedBp> pending;   // back projections of the previous block
     auto flush_bp_only = [&](size_t keep) {
         // emit BP-only passes until at most `keep` back projections are pending
@@ -699,38 +806,39 @@ void FusedPlaneEngineImpl::build_passes()
             const size_t n = std::min(pending.size() - keep, (size_t)dev::kMaxFusedBp);
             for (size_t i = 0; i < n; ++i) p.bp[p.nbp++] = pending[i];
             pending.erase(pending.begin(), pending.begin() + n);
-            passes_.push_back(p);
+            mb.passes.push_back(p);
         }
     };
+    const uint32_t nb = op.GetBlockCount();
     for (uint32_t b = 0; b < nb; ++b) {
-        const uint32_t bs = op_.getBlockSize(b);
+        const uint32_t bs = op.getBlockSize(b);
         const uint32_t parity = b & 1u;
         // long-object compensation uses the block's last geometry (compensateLongObject)
-        const ParallelScannerGeometry& last = op_.geometry(op_.getPhysicalProjectionIndex(b, bs - 1));
-        const float loc_c = use_loc_ ? last.traversal_length_through(vp.bbox_min(), vp.bbox_max()) : 0.0f;
+        const ParallelScannerGeometry& last = op.geometry(op.getPhysicalProjectionIndex(b, bs - 1));
+        const float loc_c = mb.use_loc ? last.traversal_length_through(vp.bbox_min(), vp.bbox_max()) : 0.0f;
         std::vector<dev::FusedFp> neg, pos;
         for (uint32_t p = 0; p < bs; ++p) {
-            const uint32_t phys = op_.getPhysicalProjectionIndex(b, p);
-            const ParallelScannerGeometry& g = op_.geometry(phys);
+            const uint32_t phys = op.getPhysicalProjectionIndex(b, p);
+            const ParallelScannerGeometry& g = op.geometry(phys);
             dev::FusedFp f{};
             g.source_base().store(f.src);
             g.horizontal_pitch().store(f.h);
             g.vertical_pitch().store(f.v);
             g.ray_direction().store(f.ray);
-            f.measured = op_.realProjection(phys);
+            f.measured = op.realProjection(phys);
             f.residual = image(parity, p, 0);
             f.length = image(parity, p, 1);
             f.loc_c = loc_c;
-            f.use_loc = use_loc_ ? 1 : 0;
+            f.use_loc = mb.use_loc ? 1 : 0;
             (f.ray[2] > 0.0f ? pos : neg).push_back(f);
         }
         // FP groups: rays moving towards -z (or staying in z) and towards +z
         std::vector<std::pair<int, std::vector<dev::FusedFp>>> groups;
         for (auto* grp : {&neg, &pos}) {
             const int zdir = grp == &pos ? 1 : -1;
-            for (size_t i = 0; i < grp->size(); i += (size_t)max_fp_)
+            for (size_t i = 0; i < grp->size(); i += (size_t)max_fp)
                 groups.push_back({zdir, std::vector<dev::FusedFp>(grp->begin() + i,
-                                                                  grp->begin() + std::min(grp->size(), i + (size_t)max_fp_))});
+                                                                  grp->begin() + std::min(grp->size(), i + (size_t)max_fp))});
         }
         flush_bp_only(dev::kMaxFusedBp);
         for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -739,14 +847,14 @@ void FusedPlaneEngineImpl::build_passes()
             for (const auto& f : groups[gi].second) p.fp[p.nfp++] = f;
             if (gi == 0)
                 for (const auto& e : pending) p.bp[p.nbp++] = e;
-            passes_.push_back(p);
+            mb.passes.push_back(p);
         }
         pending.clear();
         for (uint32_t p = 0; p < bs; ++p) {
-            const ParallelScannerGeometry& g = op_.geometry(op_.getPhysicalProjectionIndex(b, p));
+            const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
             dev::FusedBp e{};
             std::memcpy(e.m, g.projection_matrix().data(), sizeof(e.m));
-            e.lambda = lambda_ / (float)bs;
+            e.lambda = mb.lambda / (float)bs;
             e.residual = image(parity, p, 0);
             e.length = image(parity, p, 1);
             const std::vector<VoxelSample> smp = bp_sample_pattern(g, vp.voxel_size(), 1);
@@ -755,19 +863,193 @@ void FusedPlaneEngineImpl::build_passes()
         }
     }
     flush_bp_only(0);
-    for (const auto& p : passes_) rpt_.push_back(select_rpt(p.nfp * rows_per_plane_ * nu_));
-    passes_dev_.allocate(sizeof(dev::FusedPass) * passes_.size());
-    ETN_HIP(hipMemcpy(passes_dev_.get(), passes_.data(), sizeof(dev::FusedPass) * passes_.size(),
-                      hipMemcpyHostToDevice));
 }
 
-void FusedPlaneEngineImpl::enqueueIteration(hipStream_t stream)
+class FusedEngineImpl final : public FusedPlaneEngine {
+public:
+    FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, hipStream_t stream);
+    void enqueueIteration(hipStream_t stream) override;
+    void check(hipStream_t stream) override;
+    std::vector<LaunchTime> timeIteration(hipStream_t stream) override;
+    void autotune(hipStream_t stream) override;
+    void setTile(size_t member, int planes, int band, hipStream_t stream) override;
+    void tile(size_t member, int* planes, int* band) const override;
+    size_t memberCount() const override { return members_.size(); }
+
+private:
+    struct Launch {
+        size_t work;            // first work item
+        int groups;
+        int rpt;
+        size_t lds;
+        bool steady;
+    };
+    struct Schedule {
+        std::vector<Launch> launches;
+        DeviceBuffer work, tables;
+    };
+    void upload(hipStream_t stream);
+    void level_offsets();
+    void build_schedule(const std::vector<size_t>& ids, Schedule& out, hipStream_t stream) const;
+    void enqueue(const Schedule& s, size_t first, size_t last, hipStream_t stream, std::vector<hipEvent_t>* ev) const;
+
+    std::vector<MemberPlan> members_;
+    etn_voxel_type vt_;
+    bool vec_, cap_;
+    DeviceBuffer diag_, vols_, passes_;
+    std::vector<size_t> pass_base_;     // each member's first pass in passes_
+    Schedule sched_;
+};
+
+FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap,
+                                 hipStream_t stream)
+    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap)
 {
-    const dev::FusedPass* base = passes_dev_.as<dev::FusedPass>();
-    for (size_t i = 0; i < passes_.size(); ++i) dev::launch_fused(vt_, rpt_[i], fv_, base + i, lds_bytes_, stream);
+    diag_.allocate(sizeof(int));
+    ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
+    upload(stream);
 }
 
-void FusedPlaneEngineImpl::check(hipStream_t stream)
+// Passes and volume descriptors of every member under the current tiles, and
+// the schedule of all members.
+void FusedEngineImpl::upload(hipStream_t stream)
+{
+    std::vector<dev::FusedPass> all;
+    std::vector<dev::FusedVolume> vols;
+    pass_base_.clear();
+    for (MemberPlan& m : members_) {
+        build_passes(m);
+        pass_base_.push_back(all.size());
+        all.insert(all.end(), m.passes.begin(), m.passes.end());
+        GpuVolume* vol = m.op->getReconstructedVolume();
+        const VolumeProperties& vp = vol->properties();
+        dev::FusedVolume fv{};
+        fv.data = vol->data();
+        fv.slab = make_slab_params(vp, 0);
+        fv.nx = m.nx;
+        fv.ny = m.ny;
+        fv.nz = m.nz;
+        fv.nu = m.nu;
+        fv.nv = m.nv;
+        fv.planes = m.tile.planes;
+        fv.band = m.tile.band;
+        fv.rows_per_plane = m.rows_per_plane;
+        fv.row_lo = m.rows.as<int>();
+        fv.row_hi = m.rows.as<int>() + m.ny;
+        fv.dead_rows = m.rows.as<int>() + 2 * m.ny;
+        fv.ndead = m.ndead;
+        fv.diag = diag_.as<int>();
+        vols.push_back(fv);
+    }
+    ETN_HIP(hipStreamSynchronize(stream));   // no launch may still read the old tables
+    passes_.allocate(sizeof(dev::FusedPass) * all.size());
+    vols_.allocate(sizeof(dev::FusedVolume) * vols.size());
+    ETN_HIP(hipMemcpyAsync(passes_.get(), all.data(), passes_.bytes(), hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipMemcpyAsync(vols_.get(), vols.data(), vols_.bytes(), hipMemcpyHostToDevice, stream));
+    level_offsets();
+    std::vector<size_t> ids(members_.size());
+    std::iota(ids.begin(), ids.end(), 0);
+    build_schedule(ids, sched_, stream);
+    ETN_HIP(hipStreamSynchronize(stream));
+}
+
+// Members with fewer passes than the longest start later or earlier so that
+// the voxels a launch covers stay as even as possible across launches: each
+// member, largest total first, takes the offset whose busiest launch is
+// least loaded.
+void FusedEngineImpl::level_offsets()
+{
+    size_t length = 0;
+    for (const MemberPlan& m : members_) length = std::max(length, m.passes.size());
+    std::vector<size_t> order(members_.size());
+    std::iota(order.begin(), order.end(), 0);
+    auto voxels = [&](size_t i) { return (double)members_[i].nx * members_[i].ny * members_[i].nz; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return voxels(a) * members_[a].passes.size() > voxels(b) * members_[b].passes.size();
+    });
+    std::vector<double> load(length, 0.0);
+    for (size_t i : order) {
+        const size_t n = members_[i].passes.size();
+        size_t best = 0;
+        double best_peak = 0.0, best_sum = 0.0;
+        for (size_t o = 0; o + n <= length; ++o) {
+            double peak = 0.0, sum = 0.0;
+            for (size_t k = o; k < o + n; ++k) {
+                peak = std::max(peak, load[k]);
+                sum += load[k];
+            }
+            if (o == 0 || peak < best_peak || (peak == best_peak && sum < best_sum)) {
+                best = o;
+                best_peak = peak;
+                best_sum = sum;
+            }
+        }
+        members_[i].offset = (int)best;
+        for (size_t k = best; k < best + n; ++k) load[k] += voxels(i);
+    }
+}
+
+// Launch k carries pass k - offset of every member in `ids` that has one,
+// members with the most work per workgroup first.
+void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& out, hipStream_t stream) const
+{
+    size_t length = 0;
+    for (size_t i : ids) length = std::max(length, (size_t)members_[i].offset + members_[i].passes.size());
+    std::vector<size_t> order(ids);
+    auto group_work = [&](size_t i) {
+        const MemberPlan& m = members_[i];
+        return (double)m.tile.planes * m.nx * m.nz;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return group_work(a) > group_work(b); });
+    const dev::FusedPass* base = passes_.as<dev::FusedPass>();
+    std::vector<dev::FusedWork> work;
+    std::vector<const dev::FusedPass*> tables(length * members_.size(), nullptr);
+    out.launches.clear();
+    for (size_t k = 0; k < length; ++k) {
+        Launch l{work.size(), 0, 1, 0, true};
+        for (size_t i : order) {
+            const MemberPlan& m = members_[i];
+            if (k < (size_t)m.offset || k >= (size_t)m.offset + m.passes.size()) continue;
+            const size_t pi = k - m.offset;
+            const dev::FusedPass& p = m.passes[pi];
+            tables[k * members_.size() + i] = base + pass_base_[i] + pi;
+            for (int j = 0; j < m.ny; j += m.tile.planes) work.push_back({(int)i, j});
+            l.rpt = std::max(l.rpt, select_rpt(p.nfp * m.tile.planes * m.rows_per_plane * m.nu));
+            l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx, m.nu));
+            l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
+        }
+        l.groups = (int)(work.size() - l.work);
+        out.launches.push_back(l);
+    }
+    ETN_HIP(hipStreamSynchronize(stream));
+    out.work.allocate(sizeof(dev::FusedWork) * std::max<size_t>(work.size(), 1));
+    out.tables.allocate(sizeof(const dev::FusedPass*) * std::max<size_t>(tables.size(), 1));
+    ETN_HIP(hipMemcpyAsync(out.work.get(), work.data(), sizeof(dev::FusedWork) * work.size(), hipMemcpyHostToDevice,
+                           stream));
+    ETN_HIP(hipMemcpyAsync(out.tables.get(), tables.data(), sizeof(const dev::FusedPass*) * tables.size(),
+                           hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipStreamSynchronize(stream));
+}
+
+void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipStream_t stream,
+                              std::vector<hipEvent_t>* ev) const
+{
+    const dev::FusedWork* work = s.work.as<dev::FusedWork>();
+    const dev::FusedPass* const* tables = s.tables.as<const dev::FusedPass*>();
+    for (size_t k = first; k < last; ++k) {
+        const Launch& l = s.launches[k];
+        if (l.groups == 0) continue;
+        const dev::FusedLaunch fl{l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
+                                  tables + k * members_.size(), l.lds};
+        if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
+        dev::launch_fused(vt_, vec_, cap_, fl, stream);
+        if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
+    }
+}
+
+void FusedEngineImpl::enqueueIteration(hipStream_t stream) { enqueue(sched_, 0, sched_.launches.size(), stream, nullptr); }
+
+void FusedEngineImpl::check(hipStream_t stream)
 {
     int bad = 0;
     ETN_HIP(hipMemcpyAsync(&bad, diag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -775,32 +1057,138 @@ void FusedPlaneEngineImpl::check(hipStream_t stream)
     if (bad) throw Error(ETN_ERROR_INTERNAL, "fused plane kernel found rays outside their plane");
 }
 
-double FusedPlaneEngineImpl::timeKernels(hipStream_t stream, uint32_t* launches)
+std::vector<FusedPlaneEngine::LaunchTime> FusedEngineImpl::timeIteration(hipStream_t stream)
 {
-    // one eager iteration with an event pair around every launch; the average
-    // over the passes that carry both a BP and an FP (the steady state)
-    std::vector<hipEvent_t> ev(2 * passes_.size());
+    const size_t n = sched_.launches.size();
+    std::vector<hipEvent_t> ev(2 * n);
     for (auto& e : ev) ETN_HIP(hipEventCreate(&e));
-    const dev::FusedPass* base = passes_dev_.as<dev::FusedPass>();
-    for (size_t i = 0; i < passes_.size(); ++i) {
-        ETN_HIP(hipEventRecord(ev[2 * i], stream));
-        dev::launch_fused(vt_, rpt_[i], fv_, base + i, lds_bytes_, stream);
-        ETN_HIP(hipEventRecord(ev[2 * i + 1], stream));
-    }
+    enqueue(sched_, 0, n, stream, &ev);
     ETN_HIP(hipStreamSynchronize(stream));
-    double total = 0.0;
-    uint32_t n = 0;
-    for (size_t i = 0; i < passes_.size(); ++i) {
+    std::vector<LaunchTime> out;
+    for (size_t k = 0; k < n; ++k) {
+        if (sched_.launches[k].groups == 0) continue;
         float ms = 0.0f;
-        ETN_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-        if (passes_[i].nfp > 0 && passes_[i].nbp > 0) {
-            total += ms;
-            ++n;
-        }
+        ETN_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+        out.push_back({ms, sched_.launches[k].steady});
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    if (launches) *launches = n;
-    return n ? total / n : 0.0;
+    return out;
+}
+
+void FusedEngineImpl::setTile(size_t member, int planes, int band, hipStream_t stream)
+{
+    if (member >= members_.size()) fail("member index out of range");
+    const Tile t{planes, band};
+    if (!tile_fits(members_[member], t)) fail("tile does not fit this volume (planes must divide ny, LDS limit)");
+    members_[member].tile = t;
+    upload(stream);
+}
+
+void FusedEngineImpl::tile(size_t member, int* planes, int* band) const
+{
+    if (member >= members_.size()) fail("member index out of range");
+    *planes = members_[member].tile.planes;
+    *band = members_[member].tile.band;
+}
+
+void FusedEngineImpl::autotune(hipStream_t stream)
+{
+    // members of one size share a tile; each size is timed on its own
+    std::map<std::tuple<int, int, int, int, int, int>, std::vector<size_t>> classes;
+    for (size_t i = 0; i < members_.size(); ++i) {
+        const MemberPlan& m = members_[i];
+        classes[{m.nx, m.ny, m.nz, m.nu, m.nv, m.rows_per_plane}].push_back(i);
+    }
+    bool any = false;
+    for (auto& kv : classes) any = any || candidate_tiles(members_[kv.second[0]]).size() > 1;
+    if (!any) {
+        upload(stream);
+        return;
+    }
+    // save the volumes: the trial passes update them
+    std::vector<size_t> bytes(members_.size());
+    size_t total = 0;
+    for (size_t i = 0; i < members_.size(); ++i) {
+        bytes[i] = members_[i].op->getReconstructedVolume()->bytes();
+        total += bytes[i];
+    }
+    DeviceBuffer saved(total);
+    size_t at = 0;
+    for (size_t i = 0; i < members_.size(); ++i) {
+        ETN_HIP(hipMemcpyAsync(saved.as<char>() + at, members_[i].op->getReconstructedVolume()->data(), bytes[i],
+                               hipMemcpyDeviceToDevice, stream));
+        at += bytes[i];
+    }
+    constexpr size_t kTimed = 4;   // launches timed per candidate, after one untimed
+    hipEvent_t a, b;
+    ETN_HIP(hipEventCreate(&a));
+    ETN_HIP(hipEventCreate(&b));
+    for (auto& kv : classes) {
+        const std::vector<size_t>& ids = kv.second;
+        const std::vector<Tile> cands = candidate_tiles(members_[ids[0]]);
+        if (cands.size() <= 1) continue;
+        double best_ms = 0.0;
+        Tile best = members_[ids[0]].tile;
+        for (const Tile& t : cands) {
+            for (size_t i : ids) members_[i].tile = t;
+            upload(stream);
+            for (size_t i : ids) members_[i].offset = 0;
+            Schedule s;
+            build_schedule(ids, s, stream);
+            const size_t n = s.launches.size();
+            if (n < 2) continue;
+            const size_t last = std::min(n, 1 + kTimed);
+            enqueue(s, 0, 1, stream, nullptr);
+            ETN_HIP(hipEventRecord(a, stream));
+            enqueue(s, 1, last, stream, nullptr);
+            ETN_HIP(hipEventRecord(b, stream));
+            ETN_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            ETN_HIP(hipEventElapsedTime(&ms, a, b));
+            if (best_ms == 0.0 || ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+        for (size_t i : ids) members_[i].tile = best;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    at = 0;
+    for (size_t i = 0; i < members_.size(); ++i) {
+        ETN_HIP(hipMemcpyAsync(members_[i].op->getReconstructedVolume()->data(), saved.as<char>() + at, bytes[i],
+                               hipMemcpyDeviceToDevice, stream));
+        at += bytes[i];
+    }
+    upload(stream);
+    check(stream);
+}
+
+}  // namespace
+
+std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Member>& members, hipStream_t stream,
+                                                           std::vector<bool>* taken)
+{
+    taken->assign(members.size(), false);
+    std::vector<MemberPlan> plans;
+    etn_voxel_type vt = ETN_VOXEL_FLOAT32;
+    bool vec = true, cap = false;
+    for (size_t i = 0; i < members.size(); ++i) {
+        const etn_reconstruction_desc& d = *members[i].desc;
+        if (!plans.empty() && (d.voxel_type != vt || (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
+            continue;
+        MemberPlan m;
+        if (!plan_member(*members[i].op, d, m)) continue;
+        if (plans.empty()) {
+            vt = d.voxel_type;
+            vec = m.nx % 4 == 0;
+        }
+        cap = cap || m.nx + m.nz > 2049;
+        plans.push_back(std::move(m));
+        (*taken)[i] = true;
+    }
+    if (plans.empty()) return nullptr;
+    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, stream);
 }
 
 }  // namespace etn

@@ -1,10 +1,15 @@
 // Fused plane engine: runs whole block-iterative iterations with kernels that
-// keep one volume plane (fixed y) per workgroup and fuse the back projection
+// keep whole volume planes (fixed y) per workgroup and fuse the back projection
 // of block b-1 with the forward projection of block b in one pass over the
 // volume.  Used when every ray stays in a y plane (single-axis tilt about y).
+//
+// One engine drives any number of independent reconstructions ("members") of
+// possibly different sizes: each launch carries one pass of every member that
+// still has passes left, as a table of (member, plane group) workgroups.
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "common.hpp"
 
@@ -14,17 +19,38 @@ class BlockIterativeReconstructionOperator;
 
 class FusedPlaneEngine {
 public:
-    // Null when the reconstruction has no fused form (geometry, samples,
-    // subvolumes or mask outside what the fused kernels cover).
-    static std::unique_ptr<FusedPlaneEngine> create(BlockIterativeReconstructionOperator& op,
-                                                    const etn_reconstruction_desc& desc, hipStream_t stream);
+    struct Member {
+        BlockIterativeReconstructionOperator* op;
+        const etn_reconstruction_desc* desc;
+    };
+    struct LaunchTime {
+        float ms;
+        bool steady;            // every member pass of the launch carries a back and a forward projection
+    };
+
+    // Builds an engine over the members that have a fused form (geometry,
+    // samples, subvolumes and mask inside what the fused kernels cover) and
+    // share the first such member's voxel type and row-vector width.
+    // taken[i] tells whether members[i] is driven by the engine; null when
+    // none is.
+    static std::unique_ptr<FusedPlaneEngine> create(const std::vector<Member>& members, hipStream_t stream,
+                                                    std::vector<bool>* taken);
     virtual ~FusedPlaneEngine() = default;
     virtual void enqueueIteration(hipStream_t stream) = 0;
     // Throws when a kernel found a live ray outside the plane its row was assigned to.
     virtual void check(hipStream_t stream) = 0;
-    // One eager iteration with events around every launch: average milliseconds of
-    // the launches that carry both a back and a forward projection.
-    virtual double timeKernels(hipStream_t stream, uint32_t* launches) = 0;
+    // One eager iteration with an event pair around every launch.
+    virtual std::vector<LaunchTime> timeIteration(hipStream_t stream) = 0;
+    // Times the candidate tiles (planes per workgroup, z rows per band) of
+    // every member size on the members' own data and keeps the fastest.  The
+    // volumes are restored afterwards; tiles never change results.
+    virtual void autotune(hipStream_t stream) = 0;
+    // Index is the member's position among the engine's members (in the order
+    // given to create).  Throws ETN_ERROR_INVALID_ARGUMENT for a tile that
+    // does not fit; reallocates the launch tables (no captured graph may use them).
+    virtual void setTile(size_t member, int planes, int band, hipStream_t stream) = 0;
+    virtual void tile(size_t member, int* planes, int* band) const = 0;
+    virtual size_t memberCount() const = 0;
 };
 
 }  // namespace etn

@@ -138,6 +138,22 @@ def tilt_series(max_angle, step):
     return np.array([-max_angle + i * step for i in range(n)], np.float32)
 
 
+BATCH_EDGES = (64, 96, 128, 192, 256)
+
+
+def batch_composition(count, seed, edges=BATCH_EDGES, min_tilts=41, max_tilts=121):
+    """(edge, tilt count) of each member of a batch: the cube edge uniform over
+    `edges`, the tilt count uniform over [min_tilts, max_tilts], both drawn from
+    MT19937-64 uniforms u = (x >> 11) * 2^-53."""
+    g = MT19937_64(seed)
+    out = []
+    for _ in range(count):
+        e = edges[min(int(g.uniform() * len(edges)), len(edges) - 1)]
+        n = min_tilts + min(int(g.uniform() * (max_tilts - min_tilts + 1)), max_tilts - min_tilts)
+        out.append((e, n))
+    return out
+
+
 def add_gaussian_noise(projections, sigma_rel, seed):
     """b + N(0, (sigma_rel * max b)^2); numpy or torch input of the same kind out."""
     if isinstance(projections, np.ndarray):

@@ -158,7 +158,8 @@ typedef struct etn_reconstruction etn_reconstruction;
 
 enum {
     ETN_RECON_FLAG_GENERIC_KERNELS = 1u << 0,   /* disable the fused plane kernels */
-    ETN_RECON_FLAG_NO_GRAPH = 1u << 1           /* launch eagerly instead of replaying a hipGraph */
+    ETN_RECON_FLAG_NO_GRAPH = 1u << 1,          /* launch eagerly instead of replaying a hipGraph */
+    ETN_RECON_FLAG_NO_AUTOTUNE = 1u << 2        /* keep the default fused tile instead of timing candidates */
 };
 
 /* Parameters of BlockIterativeReconstructionOperator, named after the
@@ -213,6 +214,42 @@ etn_status etn_reconstruction_uses_fused_path(etn_reconstruction* r, int* out);
  * projection, 2 generic back projection.  Advances the reconstruction. */
 etn_status etn_reconstruction_time_kernels(etn_reconstruction* r, int* kernel, double* average_ms,
                                            uint32_t* launches);
+/* Extension (no reference counterpart): the fused kernels' tile, i.e. volume
+ * planes per workgroup and z rows per LDS band.  The first run times the
+ * candidate tiles on the reconstruction's own data unless
+ * ETN_RECON_FLAG_NO_AUTOTUNE is set; setting a tile skips that.  Tiles never
+ * change results.  ETN_ERROR_UNSUPPORTED on the generic path,
+ * ETN_ERROR_INVALID_ARGUMENT for a tile that does not fit (planes must be
+ * 1, 2, 4 or 8 and divide ny; band and detector rows must fit 64 KiB of LDS). */
+etn_status etn_reconstruction_set_tile(etn_reconstruction* r, int planes, int band);
+etn_status etn_reconstruction_tile(etn_reconstruction* r, int* planes, int* band);
+
+/* ---------------------------------------------------------------------------
+ * Batched reconstructions (extension, no reference counterpart): many
+ * independent reconstructions of different sizes advanced together.  Each
+ * fused launch carries one pass of every member that still has one, so small
+ * volumes share the GPU instead of running one after another; members with no
+ * fused form run their own kernels in the same captured iteration.
+ * ------------------------------------------------------------------------- */
+
+typedef struct etn_batch etn_batch;
+
+/* members[i] as for etn_reconstruction_create (flags there: only
+ * ETN_RECON_FLAG_GENERIC_KERNELS is honoured per member); `flags` applies to the
+ * whole batch.  All members share `stream` (NULL: a library-owned stream). */
+etn_status etn_batch_create(const etn_reconstruction_desc* members, uint32_t count, uint32_t flags, void* stream,
+                            etn_batch** out);
+etn_status etn_batch_destroy(etn_batch* b);
+/* `iterations` iterations of every member, asynchronous on the batch stream. */
+etn_status etn_batch_run_iterations(etn_batch* b, uint32_t iterations);
+etn_status etn_batch_synchronize(etn_batch* b);
+etn_status etn_batch_volume(etn_batch* b, uint32_t member, void** device_volume, size_t* bytes);
+etn_status etn_batch_residual_norm(etn_batch* b, uint32_t member, double* out);
+etn_status etn_batch_uses_fused_path(etn_batch* b, uint32_t member, int* out);
+etn_status etn_batch_tile(etn_batch* b, uint32_t member, int* planes, int* band);
+/* One more eager iteration of every member with events around every fused
+ * launch: total milliseconds of those launches and their count.  Advances the batch. */
+etn_status etn_batch_time_iteration(etn_batch* b, double* fused_ms, uint32_t* launches);
 
 #ifdef __cplusplus
 }
