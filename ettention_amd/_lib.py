@@ -139,6 +139,15 @@ SIGNATURES = [
     ("etn_batch_uses_fused_path", _i32, [_vp, _u32, C.POINTER(C.c_int)]),
     ("etn_batch_tile", _i32, [_vp, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("etn_batch_time_iteration", _i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+    ("etn_unet_create", _i32, [_u32, _u32, _u32, _vp, C.POINTER(_vp)]),
+    ("etn_unet_destroy", _i32, [_vp]),
+    ("etn_unet_parameter_count", _i32, [C.POINTER(C.c_size_t)]),
+    ("etn_unet_set_parameters", _i32, [_vp, _vp, C.c_size_t]),
+    ("etn_unet_get_parameters", _i32, [_vp, _vp, C.c_size_t]),
+    ("etn_unet_predict", _i32, [_vp, _vp, _f32, _vp]),
+    ("etn_unet_gradient", _i32, [_vp, _vp, _vp, _f32, _vp, C.POINTER(C.c_double)]),
+    ("etn_unet_train_step", _i32, [_vp, _vp, _vp, _f32, _f32, C.POINTER(C.c_double)]),
+    ("etn_unet_time_forward", _i32, [_vp, _vp, _f32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
 _lib = None

@@ -21,11 +21,13 @@ OBJ = os.path.join(PKG, "_build")
 SOURCES = [
     "projectors.hip",
     "fused.hip",
+    "unet_kernels.hip",
     "geometry.cpp",
     "projection_set.cpp",
     "operators.cpp",
     "block_iterative.cpp",
     "batch.cpp",
+    "unet.cpp",
     "capi.cpp",
 ]
 

@@ -8,6 +8,7 @@
 #include "batch.hpp"
 #include "block_iterative.hpp"
 #include "etn/etn.h"
+#include "unet.hpp"
 
 namespace {
 
@@ -55,6 +56,18 @@ struct etn_reconstruction {
 
 struct etn_batch {
     std::unique_ptr<etn::BatchReconstruction> batch;
+};
+
+struct etn_unet {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::unique_ptr<etn::UNet3D> net;
+    ~etn_unet()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        net.reset();
+        if (own_stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 extern "C" {
@@ -452,6 +465,99 @@ etn_status etn_batch_time_iteration(etn_batch* b, double* fused_ms, uint32_t* la
         need(fused_ms, "fused_ms");
         need(launches, "launches");
         *fused_ms = b->batch->timeIteration(launches);
+    });
+}
+
+etn_status etn_unet_create(uint32_t nx, uint32_t ny, uint32_t nz, void* stream, etn_unet** out)
+{
+    return guarded([&] {
+        need(out, "output");
+        *out = nullptr;
+        auto n = std::make_unique<etn_unet>();
+        if (stream) {
+            n->stream = as_stream(stream);
+        } else {
+            ETN_HIP(hipStreamCreate(&n->stream));
+            n->own_stream = true;
+        }
+        n->net = std::make_unique<etn::UNet3D>((int)nx, (int)ny, (int)nz, n->stream);
+        *out = n.release();
+    });
+}
+
+etn_status etn_unet_destroy(etn_unet* n)
+{
+    return guarded([&] { delete n; });
+}
+
+etn_status etn_unet_parameter_count(size_t* out)
+{
+    return guarded([&] {
+        need(out, "output");
+        *out = etn::UNet3D::parameterCount();
+    });
+}
+
+etn_status etn_unet_set_parameters(etn_unet* n, const float* host, size_t count)
+{
+    return guarded([&] {
+        need(n, "network");
+        need(host, "parameters");
+        n->net->setParameters(host, count);
+    });
+}
+
+etn_status etn_unet_get_parameters(etn_unet* n, float* host, size_t count)
+{
+    return guarded([&] {
+        need(n, "network");
+        need(host, "parameters");
+        n->net->getParameters(host, count);
+    });
+}
+
+etn_status etn_unet_predict(etn_unet* n, const float* x, float scale, float* out)
+{
+    return guarded([&] {
+        need(n, "network");
+        need(x, "input volume");
+        need(out, "output volume");
+        n->net->predict(x, scale, out);
+    });
+}
+
+etn_status etn_unet_gradient(etn_unet* n, const float* x, const float* target, float scale, float* grad_host,
+                             double* loss)
+{
+    return guarded([&] {
+        need(n, "network");
+        need(x, "input volume");
+        need(target, "target volume");
+        const double l = n->net->gradient(x, target, scale);
+        if (loss) *loss = l;
+        if (grad_host) n->net->copyGradient(grad_host, etn::UNet3D::parameterCount());
+    });
+}
+
+etn_status etn_unet_train_step(etn_unet* n, const float* x, const float* target, float scale, float lr,
+                               double* loss)
+{
+    return guarded([&] {
+        need(n, "network");
+        need(x, "input volume");
+        need(target, "target volume");
+        const double l = n->net->trainStep(x, target, scale, lr);
+        if (loss) *loss = l;
+    });
+}
+
+etn_status etn_unet_time_forward(etn_unet* n, const float* x, float scale, double* ms, double* flops)
+{
+    return guarded([&] {
+        need(n, "network");
+        need(x, "input volume");
+        need(ms, "ms");
+        *ms = n->net->timeForward(x, scale, flops);
     });
 }
 

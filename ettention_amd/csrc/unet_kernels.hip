@@ -1,0 +1,620 @@
+// Learned correction network kernels (see unet_kernels.hpp).
+//
+// The 3x3x3 convolution is an implicit GEMM on the matrix cores: output
+// voxels are the GEMM rows, output channels the columns, and the reduction
+// runs over k = tap * cin + ci.  Each wave owns MT tiles of 16 voxels and all
+// cout channels (NT tiles of 16); per 32-wide k step a lane loads its 8
+// channels of one tap for each voxel tile (one 16-byte load from the
+// channels-last activation, or zeros outside the volume) and 8 weights per
+// channel tile, then issues MT x NT v_mfma_f32_16x16x32_bf16.  Lane layouts
+// (checked on the device): A[i][k] in lane i + 16 (k / 8), element k % 8;
+// B[k][j] in lane j + 16 (k / 8); D[i][j] in lane j + 16 (i / 4), element i % 4.
+//
+// The weight gradient is the GEMM dW[co][(tap, ci)] = sum_m gy[m][co] x[m + d(tap)][ci]
+// with the voxels as the reduction: a workgroup takes one tap and a chunk of
+// voxels, stages gy and the tap-shifted input transposed in LDS (32 voxels at
+// a time) and accumulates all (co, ci) tiles; the per-chunk partials are then
+// summed in a fixed order, so every gradient is deterministic.
+#include <algorithm>
+
+#include "common.hpp"
+#include "device_math.hpp"
+#include "unet_kernels.hpp"
+
+namespace etn {
+namespace nn {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) { return dev::BF16::pack(f); }
+
+__device__ __forceinline__ u32x4 load16(const uint16_t* p) { return *reinterpret_cast<const u32x4*>(p); }
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+constexpr int kConvWaves = 4;
+constexpr int kThreads = 256;
+
+// ---- 3x3x3 convolution, implicit GEMM ---------------------------------------
+template <int NT, int MT, bool kCin1>
+__global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const size_t M = (size_t)a.D * a.H * a.W;
+    const size_t m0 = ((size_t)blockIdx.x * kConvWaves + wave) * (MT * 16);
+    if (m0 >= M) return;   // the whole wave is past the volume (no barriers below)
+    const int HW = a.H * a.W;
+    int vz[MT], vy[MT], vx[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const size_t m = m0 + mt * 16 + row;
+        if (m < M) {
+            vz[mt] = (int)(m / HW);
+            const int r = (int)(m - (size_t)vz[mt] * HW);
+            vy[mt] = r / a.W;
+            vx[mt] = r - vy[mt] * a.W;
+        } else {
+            vz[mt] = -4;   // every tap falls outside
+            vy[mt] = 0;
+            vx[mt] = 0;
+        }
+    }
+    floatx4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const int ca = a.src.ca, cin = kCin1 ? 1 : a.src.ca + a.src.cb;
+    const int Hh = a.H >> 1, Wh = a.W >> 1;
+    const bf16x8 zero = as_bf16x8(u32x4{0u, 0u, 0u, 0u});
+    for (int k0 = kg * 8; k0 < a.kpad; k0 += 32) {
+        bf16x8 B[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) B[nt] = as_bf16x8(load16(a.wt + (size_t)(nt * 16 + row) * a.kpad + k0));
+        bf16x8 A[MT];
+        if (kCin1) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                uint16_t v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int t = k0 + e;
+                    v[e] = 0;
+                    if (t < 27) {
+                        const int zz = vz[mt] + t / 9 - 1, yy = vy[mt] + (t / 3) % 3 - 1, xx = vx[mt] + t % 3 - 1;
+                        if ((unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W)
+                            v[e] = a.src.a[((size_t)zz * a.H + yy) * a.W + xx];
+                    }
+                }
+                A[mt] = as_bf16x8(u32x4{v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16),
+                                        v[4] | ((uint32_t)v[5] << 16), v[6] | ((uint32_t)v[7] << 16)});
+            }
+        } else {
+            const int t = k0 / cin, ci = k0 - t * cin;
+            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int zz = vz[mt] + dz, yy = vy[mt] + dy, xx = vx[mt] + dx;
+                A[mt] = zero;
+                if (t < 27 && (unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H &&
+                    (unsigned)xx < (unsigned)a.W) {
+                    const uint16_t* p;
+                    if (ci < ca) {
+                        const size_t v = a.src.a_half ? ((size_t)(zz >> 1) * Hh + (yy >> 1)) * Wh + (xx >> 1)
+                                                      : ((size_t)zz * a.H + yy) * a.W + xx;
+                        p = a.src.a + v * ca + ci;
+                    } else {
+                        p = a.src.b + (((size_t)zz * a.H + yy) * a.W + xx) * a.src.cb + (ci - ca);
+                    }
+                    A[mt] = as_bf16x8(load16(p));
+                }
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B[nt], acc[mt][nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = nt * 16 + row;
+            const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t m = m0 + mt * 16 + kg * 4 + r;
+                if (m < M) {
+                    float v = acc[mt][nt][r] + b;
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    a.out[m * a.cout + co] = f2bf(v);
+                }
+            }
+        }
+}
+
+template <int NT, int MT>
+void launch_conv(const ConvArgs& a, hipStream_t stream)
+{
+    const size_t M = (size_t)a.D * a.H * a.W;
+    const size_t per_block = (size_t)kConvWaves * MT * 16;
+    const dim3 grid((unsigned)((M + per_block - 1) / per_block));
+    if (a.src.ca == 1 && a.src.cb == 0)
+        conv3_fwd_kernel<NT, MT, true><<<grid, kThreads, 0, stream>>>(a);
+    else
+        conv3_fwd_kernel<NT, MT, false><<<grid, kThreads, 0, stream>>>(a);
+}
+
+// ---- weight gradient ----------------------------------------------------------
+constexpr int kLdsRow = 40;        // 32 voxels + 8 pad (bf16), 80-byte rows keep 16-byte alignment
+constexpr int kMaxCout = 64;
+constexpr int kMaxCin = 96;
+
+struct WgradArgs {
+    ConvSrc src;
+    int D, H, W;
+    const uint16_t* gy;
+    int cout;
+    int chunk;          // voxels per workgroup, multiple of 32
+    float* partial;     // [chunks][27][cout][cin16]
+};
+
+template <bool kCin1>
+__global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t gyT[kMaxCout * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t xT[kMaxCin * kLdsRow];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int t = blockIdx.y;
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const size_t M = (size_t)w.D * w.H * w.W;
+    const int HW = w.H * w.W;
+    const int ca = w.src.ca, cin = kCin1 ? 1 : w.src.ca + w.src.cb;
+    const int cin16 = (cin + 15) / 16 * 16;
+    const int nco = w.cout / 16, nci = cin16 / 16, ntiles = nco * nci;
+    const int Hh = w.H >> 1, Wh = w.W >> 1;
+    const size_t mbeg = (size_t)blockIdx.x * w.chunk;
+    if (kCin1) {
+        for (int i = threadIdx.x; i < 16 * kLdsRow; i += kThreads) xT[i] = 0;
+    }
+    floatx4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int s = 0; s < w.chunk; s += 32) {
+        const size_t ms = mbeg + s;
+        // gy rows of 32 voxels, transposed: gyT[co][voxel]
+        const int gpv = w.cout / 8;
+        for (int g = threadIdx.x; g < 32 * gpv; g += kThreads) {
+            const int mi = g / gpv, c0 = (g - mi * gpv) * 8;
+            const size_t m = ms + mi;
+            u32x4 v = u32x4{0u, 0u, 0u, 0u};
+            if (m < M) v = load16(w.gy + m * w.cout + c0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                gyT[(c0 + 2 * e) * kLdsRow + mi] = (uint16_t)(v[e] & 0xffffu);
+                gyT[(c0 + 2 * e + 1) * kLdsRow + mi] = (uint16_t)(v[e] >> 16);
+            }
+        }
+        // tap-shifted input, transposed: xT[ci][voxel]
+        if (kCin1) {
+            for (int mi = threadIdx.x; mi < 32; mi += kThreads) {
+                const size_t m = ms + mi;
+                uint16_t v = 0;
+                if (m < M) {
+                    const int z = (int)(m / HW), r = (int)(m - (size_t)z * HW), y = r / w.W, x = r - y * w.W;
+                    const int zz = z + dz, yy = y + dy, xx = x + dx;
+                    if ((unsigned)zz < (unsigned)w.D && (unsigned)yy < (unsigned)w.H && (unsigned)xx < (unsigned)w.W)
+                        v = w.src.a[((size_t)zz * w.H + yy) * w.W + xx];
+                }
+                xT[mi] = v;
+            }
+        } else {
+            const int gpx = cin / 8;
+            for (int g = threadIdx.x; g < 32 * gpx; g += kThreads) {
+                const int mi = g / gpx, c0 = (g - mi * gpx) * 8;
+                const size_t m = ms + mi;
+                u32x4 v = u32x4{0u, 0u, 0u, 0u};
+                if (m < M) {
+                    const int z = (int)(m / HW), r = (int)(m - (size_t)z * HW), y = r / w.W, x = r - y * w.W;
+                    const int zz = z + dz, yy = y + dy, xx = x + dx;
+                    if ((unsigned)zz < (unsigned)w.D && (unsigned)yy < (unsigned)w.H && (unsigned)xx < (unsigned)w.W) {
+                        if (c0 < ca) {
+                            const size_t vi = w.src.a_half ? ((size_t)(zz >> 1) * Hh + (yy >> 1)) * Wh + (xx >> 1)
+                                                           : ((size_t)zz * w.H + yy) * w.W + xx;
+                            v = load16(w.src.a + vi * ca + c0);
+                        } else {
+                            v = load16(w.src.b + (((size_t)zz * w.H + yy) * w.W + xx) * w.src.cb + (c0 - ca));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    xT[(c0 + 2 * e) * kLdsRow + mi] = (uint16_t)(v[e] & 0xffffu);
+                    xT[(c0 + 2 * e + 1) * kLdsRow + mi] = (uint16_t)(v[e] >> 16);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int tile = wave + 4 * i;
+            if (tile < ntiles) {
+                const int ct = tile / nci, it = tile - ct * nci;
+                const bf16x8 A = as_bf16x8(*reinterpret_cast<const u32x4*>(&gyT[(ct * 16 + row) * kLdsRow + kg * 8]));
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const u32x4*>(&xT[(it * 16 + row) * kLdsRow + kg * 8]));
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int tile = wave + 4 * i;
+        if (tile < ntiles) {
+            const int ct = tile / nci, it = tile - ct * nci;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = ct * 16 + kg * 4 + r, ci = it * 16 + row;
+                w.partial[(((size_t)blockIdx.x * 27 + t) * w.cout + co) * cin16 + ci] = acc[i][r];
+            }
+        }
+    }
+}
+
+__global__ void wgrad_reduce_kernel(const float* partial, int chunks, int cout, int cin, int cin16, float* dw)
+{
+    const int n = cout * 27 * cin;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int co = i / (27 * cin), r = i - co * 27 * cin, t = r / cin, ci = r - t * cin;
+    const size_t stride = (size_t)27 * cout * cin16;
+    const float* p = partial + ((size_t)t * cout + co) * cin16 + ci;
+    float s = 0.0f;
+    for (int c = 0; c < chunks; ++c) s += p[c * stride];
+    dw[i] = s;
+}
+
+__global__ void channel_sum_partial_kernel(const uint16_t* g, size_t voxels, int C, int chunk, float* partial)
+{
+    __shared__ float red[kThreads];
+    const int lanes = kThreads / C;   // voxel lanes per channel
+    const int ch = threadIdx.x % C, j = threadIdx.x / C;
+    const size_t beg = (size_t)blockIdx.x * chunk, end = std::min(voxels, beg + chunk);
+    float s = 0.0f;
+    if (j < lanes)
+        for (size_t m = beg + j; m < end; m += lanes) s += bf2f(g[m * C + ch]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.0f;
+        for (int q = 0; q < lanes; ++q) t += red[q * C + threadIdx.x];
+        partial[(size_t)blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+
+__global__ void sum_partials_kernel(const float* partial, int chunks, int C, float* out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float s = 0.0f;
+    for (int k = 0; k < chunks; ++k) s += partial[(size_t)k * C + c];
+    out[c] = s;
+}
+
+__global__ void pack_weights_kernel(const float* w, int cout, int cin, uint16_t* wt, int kpad, uint16_t* wb,
+                                    int kpad_b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cout * kpad) {
+        const int co = i / kpad, k = i - co * kpad, t = k / cin, ci = k - t * cin;
+        wt[i] = t < 27 ? f2bf(w[((size_t)co * 27 + t) * cin + ci]) : (uint16_t)0;
+    }
+    if (wb && i < cin * kpad_b) {
+        const int ci = i / kpad_b, k = i - ci * kpad_b, t = k / cout, co = k - t * cout;
+        wb[i] = t < 27 ? f2bf(w[((size_t)co * 27 + (26 - t)) * cin + ci]) : (uint16_t)0;
+    }
+}
+
+__global__ void relu_backward_kernel(uint16_t* g, const uint16_t* y, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint16_t v = y[i];
+        if ((v & 0x8000u) || (v & 0x7fffu) == 0) g[i] = 0;
+    }
+}
+
+__global__ void pool2_kernel(const uint16_t* x, int D, int H, int W, int C, uint16_t* y)
+{
+    const int Dl = D / 2, Hl = H / 2, Wl = W / 2;
+    const size_t n = (size_t)Dl * Hl * Wl * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t p = i / C;
+        const int xl = (int)(p % Wl), yl = (int)((p / Wl) % Hl), zl = (int)(p / ((size_t)Wl * Hl));
+        float s = 0.0f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int e = 0; e < 2; ++e)
+                    s += bf2f(x[((((size_t)(2 * zl + a) * H) + 2 * yl + b) * W + 2 * xl + e) * C + c]);
+        y[i] = f2bf(s * 0.125f);
+    }
+}
+
+__global__ void pool2_backward_kernel(const uint16_t* gl, int D, int H, int W, int C, uint16_t* g, int accumulate)
+{
+    const size_t n = (size_t)D * H * W * C;
+    const int Hl = H / 2, Wl = W / 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t m = i / C;
+        const int x = (int)(m % W), y = (int)((m / W) % H), z = (int)(m / ((size_t)W * H));
+        const float v = bf2f(gl[(((size_t)(z / 2) * Hl + y / 2) * Wl + x / 2) * C + c]) * 0.125f;
+        g[i] = f2bf(accumulate ? bf2f(g[i]) + v : v);
+    }
+}
+
+__global__ void up2_backward_kernel(const uint16_t* g, int stride, int c0, int D, int H, int W, int C, uint16_t* gl)
+{
+    const int Dl = D / 2, Hl = H / 2, Wl = W / 2;
+    const size_t n = (size_t)Dl * Hl * Wl * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t p = i / C;
+        const int xl = (int)(p % Wl), yl = (int)((p / Wl) % Hl), zl = (int)(p / ((size_t)Wl * Hl));
+        float s = 0.0f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int e = 0; e < 2; ++e)
+                    s += bf2f(g[((((size_t)(2 * zl + a) * H) + 2 * yl + b) * W + 2 * xl + e) * stride + c0 + c]);
+        gl[i] = f2bf(s);
+    }
+}
+
+__global__ void take_channels_kernel(const uint16_t* g, int stride, int c0, size_t voxels, int C, uint16_t* dst,
+                                     int accumulate)
+{
+    const size_t n = voxels * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = i / C;
+        const int c = (int)(i - m * C);
+        const float v = bf2f(g[m * stride + c0 + c]);
+        dst[i] = f2bf(accumulate ? bf2f(dst[i]) + v : v);
+    }
+}
+
+__global__ void to_bf16_scaled_kernel(const float* x, float scale, size_t n, uint16_t* y)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = f2bf(x[i] * scale);
+}
+
+__global__ void head_forward_kernel(const uint16_t* d1, const float* x, float scale, const float* hw, const float* hb,
+                                    size_t n, float* pred)
+{
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (size_t)gridDim.x * blockDim.x) {
+        const u32x4 a = load16(d1 + m * 16), b = load16(d1 + m * 16 + 8);
+        float s = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            s += hw[2 * e] * bf2f((uint16_t)(a[e] & 0xffffu)) + hw[2 * e + 1] * bf2f((uint16_t)(a[e] >> 16));
+            s += hw[8 + 2 * e] * bf2f((uint16_t)(b[e] & 0xffffu)) + hw[9 + 2 * e] * bf2f((uint16_t)(b[e] >> 16));
+        }
+        pred[m] = x[m] * scale + (s + hb[0]);
+    }
+}
+
+__global__ void unscale_kernel(const float* pred, const float* x, float scale, size_t n, float* out)
+{
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (size_t)gridDim.x * blockDim.x)
+        out[m] = x[m] + (pred[m] - x[m] * scale) / scale;
+}
+
+constexpr int kRedBlocks = 1024;
+
+__global__ void mse_partial_kernel(const float* pred, const float* target, float scale, size_t n, float* g,
+                                   double* partial)
+{
+    __shared__ double red[kThreads];
+    double s = 0.0;
+    const float inv = 2.0f / (float)n;
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (size_t)gridDim.x * blockDim.x) {
+        const float r = pred[m] - target[m] * scale;
+        g[m] = r * inv;
+        s += (double)r * (double)r;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w /= 2) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void mse_final_kernel(const double* partial, int blocks, size_t n, double* loss)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < blocks; ++b) s += partial[b];
+        *loss = s / (double)n;
+    }
+}
+
+__global__ void head_backward_kernel(const uint16_t* d1, const float* gp, const float* hw, size_t n, uint16_t* gd1,
+                                     float* partial)
+{
+    __shared__ float red[kThreads * 17];
+    float s[17];
+#pragma unroll
+    for (int c = 0; c < 17; ++c) s[c] = 0.0f;
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (size_t)gridDim.x * blockDim.x) {
+        const float g = gp[m];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            s[c] += g * bf2f(d1[m * 16 + c]);
+            gd1[m * 16 + c] = f2bf(g * hw[c]);
+        }
+        s[16] += g;
+    }
+#pragma unroll
+    for (int c = 0; c < 17; ++c) red[c * kThreads + threadIdx.x] = s[c];
+    __syncthreads();
+    if (threadIdx.x < 17) {
+        float t = 0.0f;
+        for (int q = 0; q < kThreads; ++q) t += red[threadIdx.x * kThreads + q];
+        partial[(size_t)blockIdx.x * 17 + threadIdx.x] = t;
+    }
+}
+
+__global__ void adam_kernel(float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+                            float eps, float bc1, float bc2)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+
+unsigned grid_for(size_t n)
+{
+    return (unsigned)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 65536));
+}
+
+}  // namespace
+
+void conv3_forward(const ConvArgs& a, hipStream_t stream)
+{
+    const bool cin1 = a.src.ca == 1 && a.src.cb == 0;
+    if (!cin1 && ((a.src.ca % 8) || (a.src.cb % 8))) fail("conv3: input channels must be multiples of 8");
+    if (a.kpad != kpad_for(cin1 ? 1 : a.src.ca + a.src.cb)) fail("conv3: weight row length mismatch");
+    if (a.src.a_half && ((a.D | a.H | a.W) & 1)) fail("conv3: upsampled input needs even sizes");
+    switch (a.cout) {
+        case 16: launch_conv<1, 4>(a, stream); break;
+        case 32: launch_conv<2, 4>(a, stream); break;
+        case 48: launch_conv<3, 2>(a, stream); break;
+        case 64: launch_conv<4, 2>(a, stream); break;
+        case 96: launch_conv<6, 2>(a, stream); break;
+        default: fail("conv3: output channels must be 16, 32, 48, 64 or 96");
+    }
+    hip_check(hipGetLastError(), "conv3 forward launch");
+}
+
+void conv3_weight_grad(const ConvSrc& src, int D, int H, int W, const uint16_t* gy, int cout, float* partial,
+                       hipStream_t stream)
+{
+    const bool cin1 = src.ca == 1 && src.cb == 0;
+    if (cout % 16 || cout > kMaxCout || src.ca + src.cb > kMaxCin) fail("conv3 weight gradient: channel counts");
+    WgradArgs w{src, D, H, W, gy, cout, kWgradChunk, partial};
+    const size_t M = (size_t)D * H * W;
+    const dim3 grid((unsigned)wgrad_chunks(M), 27);
+    if (cin1) conv3_wgrad_kernel<true><<<grid, kThreads, 0, stream>>>(w);
+    else conv3_wgrad_kernel<false><<<grid, kThreads, 0, stream>>>(w);
+    hip_check(hipGetLastError(), "conv3 weight gradient launch");
+}
+
+void conv3_weight_grad_reduce(const float* partial, int chunks, int cout, int cin, float* dw, hipStream_t stream)
+{
+    const int n = cout * 27 * cin;
+    const int cin16 = (cin + 15) / 16 * 16;
+    wgrad_reduce_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, stream>>>(partial, chunks, cout, cin, cin16, dw);
+    hip_check(hipGetLastError(), "weight gradient reduction launch");
+}
+
+void channel_sum(const uint16_t* g, size_t voxels, int channels, float* partial, float* out, hipStream_t stream)
+{
+    if (channels > kThreads || kThreads % channels) fail("channel_sum: channel count");
+    const int chunks = wgrad_chunks(voxels);
+    channel_sum_partial_kernel<<<chunks, kThreads, 0, stream>>>(g, voxels, channels, kWgradChunk, partial);
+    sum_partials_kernel<<<1, 128, 0, stream>>>(partial, chunks, channels, out);
+    hip_check(hipGetLastError(), "channel sum launch");
+}
+
+void pack_weights(const float* w, int cout, int cin, uint16_t* wt, uint16_t* wb, hipStream_t stream)
+{
+    const int kpad = kpad_for(cin), kpad_b = kpad_for(cout);
+    const int n = std::max(cout * kpad, wb ? cin * kpad_b : 0);
+    pack_weights_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, stream>>>(w, cout, cin, wt, kpad, wb, kpad_b);
+    hip_check(hipGetLastError(), "weight packing launch");
+}
+
+void relu_backward(uint16_t* g, const uint16_t* y, size_t count, hipStream_t stream)
+{
+    relu_backward_kernel<<<grid_for(count), kThreads, 0, stream>>>(g, y, count);
+}
+
+void pool2(const uint16_t* x, int D, int H, int W, int C, uint16_t* y, hipStream_t stream)
+{
+    pool2_kernel<<<grid_for((size_t)D * H * W * C / 8), kThreads, 0, stream>>>(x, D, H, W, C, y);
+}
+
+void pool2_backward(const uint16_t* g_low, int D, int H, int W, int C, uint16_t* g_full, int accumulate,
+                    hipStream_t stream)
+{
+    pool2_backward_kernel<<<grid_for((size_t)D * H * W * C), kThreads, 0, stream>>>(g_low, D, H, W, C, g_full,
+                                                                                    accumulate);
+}
+
+void up2_backward(const uint16_t* g, int stride, int c0, int D, int H, int W, int C, uint16_t* g_low,
+                  hipStream_t stream)
+{
+    up2_backward_kernel<<<grid_for((size_t)D * H * W * C / 8), kThreads, 0, stream>>>(g, stride, c0, D, H, W, C,
+                                                                                      g_low);
+}
+
+void take_channels(const uint16_t* g, int stride, int c0, size_t voxels, int C, uint16_t* dst, int accumulate,
+                   hipStream_t stream)
+{
+    take_channels_kernel<<<grid_for(voxels * C), kThreads, 0, stream>>>(g, stride, c0, voxels, C, dst, accumulate);
+}
+
+void to_bf16_scaled(const float* x, float scale, size_t count, uint16_t* y, hipStream_t stream)
+{
+    to_bf16_scaled_kernel<<<grid_for(count), kThreads, 0, stream>>>(x, scale, count, y);
+}
+
+void head_forward(const uint16_t* d1, const float* x, float scale, const float* hw, const float* hb, size_t voxels,
+                  float* pred, hipStream_t stream)
+{
+    head_forward_kernel<<<grid_for(voxels), kThreads, 0, stream>>>(d1, x, scale, hw, hb, voxels, pred);
+}
+
+void unscale_prediction(const float* pred, const float* x, float scale, size_t voxels, float* out, hipStream_t stream)
+{
+    unscale_kernel<<<grid_for(voxels), kThreads, 0, stream>>>(pred, x, scale, voxels, out);
+}
+
+void mse_loss(const float* pred, const float* target, float scale, size_t voxels, float* g_pred, double* partial,
+              double* loss, hipStream_t stream)
+{
+    mse_partial_kernel<<<kRedBlocks, kThreads, 0, stream>>>(pred, target, scale, voxels, g_pred, partial);
+    mse_final_kernel<<<1, 64, 0, stream>>>(partial, kRedBlocks, voxels, loss);
+}
+
+void head_backward(const uint16_t* d1, const float* g_pred, const float* hw, size_t voxels, uint16_t* g_d1,
+                   float* partial, float* ghw_ghb, hipStream_t stream)
+{
+    head_backward_kernel<<<kRedBlocks, kThreads, 0, stream>>>(d1, g_pred, hw, voxels, g_d1, partial);
+    sum_partials_kernel<<<1, 64, 0, stream>>>(partial, kRedBlocks, 17, ghw_ghb);
+}
+
+void adam_step(float* p, const float* g, float* m, float* v, size_t count, float lr, float beta1, float beta2,
+               float eps, float bc1, float bc2, hipStream_t stream)
+{
+    adam_kernel<<<grid_for(count), kThreads, 0, stream>>>(p, g, m, v, count, lr, beta1, beta2, eps, bc1, bc2);
+}
+
+}  // namespace nn
+}  // namespace etn
