@@ -37,6 +37,15 @@ WORKLOADS = {
     "c4": dict(members=64, algorithm="sart", order="random", lam=0.3,
                name="batch of 64 SART reconstructions, cube edges uniform over {64,96,128,192,256}, "
                     "tilt counts uniform over [41,121] at +-60 deg, detector = edge^2"),
+    # time to the residual plain FP32 SART reaches after 50 iterations: bf16 SART + learned jump vs plain
+    # precision variant: the same SART in f32, bf16 and f16 storage, error per iteration against f32
+    "c3": dict(res=(1024, 1024, 256), det=(1024, 1024), max_angle=60.0, step=1.0, iterations=10, lam=0.3,
+               order="random",
+               name="precision variant: SART on 1024x1024x256, 121 tilts +-60 deg by 1 deg, 1024^2 detector, "
+                    "f32 vs bf16 and f16 storage, 10 iterations"),
+    "c5": dict(res=512, tilts=121, k0=3, k_max=50, crop=128, lam=0.3, order="random",
+               name="time-to-target-residual at 512^3, 121 tilts +-60 deg, Poisson 1e4 counts: bf16 SART with a "
+                    "learned jump from iterate 3 vs plain FP32 SART to its lowest residual within 50 iterations"),
 }
 
 
@@ -55,6 +64,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-projections", type=int, default=32,
                     help="projection steps the CPU baseline times (about 0.35 s each on 16 threads)")
+    ap.add_argument("--train-phantoms", type=int, default=6, help="c5: training phantoms (512^3 each)")
+    ap.add_argument("--train-steps", type=int, default=600, help="c5: Adam steps on 128^3 crops")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (traffic calibration for rocprof counters)")
     args = ap.parse_args()
@@ -66,6 +77,12 @@ def main():
 
     if args.workload == "c4":
         run_batch(args)
+        return
+    if args.workload == "c5":
+        run_learned(args)
+        return
+    if args.workload == "c3":
+        run_precision(args)
         return
 
     import torch
@@ -161,6 +178,88 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def run_precision(args):
+    """SART with f32, bf16 and f16 voxel storage on the same input: GUPS of
+    each and, after every iteration, the relative L2 distance of the 16-bit
+    volumes from the f32 volume (all FP32 arithmetic), plus the FP64 residual
+    norm of each."""
+    import torch
+    import ettention_amd as E
+    from ettention_amd import api as A
+    from ettention_amd import synthetic as S
+
+    wl = WORKLOADS["c3"]
+    nx, ny, nz = wl["res"]
+    nu, nv = wl["det"]
+    tilts = S.tilt_series(wl["max_angle"], wl["step"])
+    N = len(tilts)
+    vg = E.VolumeGeometry.make((nx, ny, nz))
+    base = E.ScannerGeometry.mrc_base(nu, nv)
+    dev = torch.device("cuda:0")
+    phantom = S.ellipsoid_phantom((nz, ny, nx), seed=1003, device=dev)
+    projs = torch.zeros((N, nv, nu), dtype=torch.float32, device=dev)
+    scratch = torch.zeros((nv, nu), dtype=torch.float32, device=dev)
+    for i, t in enumerate(tilts):
+        scratch.zero_()
+        A.forward_project(vg, phantom, A.rotate(base, float(t)), projs[i], scratch)
+    projs = S.add_gaussian_noise(projs, 0.02, seed=7)
+    del scratch
+    ops = {vt: A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                                      projections=projs, algorithm="sart", lam=wl["lam"],
+                                                      projection_iteration=wl["order"], voxel_type=vt)
+           for vt in ("float", "bfloat16", "half")}
+    per_iter = {vt: [] for vt in ops}
+    elapsed = {vt: 0.0 for vt in ops}
+    for vt, r in ops.items():      # tile autotuning and graph capture outside the timed region
+        r.run_iterations(1)
+        r.set_volume(torch.zeros(r.shape, dtype={"float": torch.float32, "bfloat16": torch.bfloat16,
+                                                 "half": torch.float16}[vt], device=dev))
+    for it in range(1, wl["iterations"] + 1):
+        vols = {}
+        for vt, r in ops.items():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r.run_iterations(1)
+            r.synchronize()
+            elapsed[vt] += time.perf_counter() - t0
+            vols[vt] = r.get_reconstructed_volume(dtype=torch.float32)
+        ref = vols["float"]
+        rn = float(torch.linalg.vector_norm(ref.double()))
+        row = {"iteration": it}
+        for vt in ("bfloat16", "half"):
+            row[vt + "_rel_l2_vs_f32"] = float(torch.linalg.vector_norm((vols[vt] - ref).double())) / rn
+        per_iter["float"].append(row)
+        log("iteration %d: rel L2 vs f32  bf16 %.3e  f16 %.3e" % (it, row["bfloat16_rel_l2_vs_f32"],
+                                                                 row["half_rel_l2_vs_f32"]))
+        del vols, ref
+    norms = {vt: r.residual_norm() for vt, r in ops.items()}
+    V = nx * ny * nz
+    gups = {vt: V * N * wl["iterations"] / elapsed[vt] / 1e9 for vt in ops}
+    log("GUPS: " + ", ".join("%s %.1f" % kv for kv in gups.items()))
+    out = {
+        "metric": "SART GUPS, bf16 storage, 1024x1024x256 x 121 tilts (precision variant)",
+        "value": round(gups["bfloat16"], 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": wl["iterations"],
+        "warmup": 1,
+        "ms_per_step": round(1e3 * elapsed["bfloat16"] / wl["iterations"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-storage/f32-accumulate",
+        "data": "synthetic (seeded ellipsoid phantom, FP projections + 2% Gaussian noise)",
+        "config": {"workload": wl["name"], "volume": [nx, ny, nz], "detector": [nu, nv], "projections": N,
+                   "parallelism": "single GPU"},
+        "gups": {vt: round(g, 3) for vt, g in gups.items()},
+        "residual_norm_after": norms,
+        "rel_l2_vs_f32_per_iteration": per_iter["float"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args):
     """The batched workload: GUPS = sum over members of voxels x projections,
     per iteration, over the wall time of the iterations."""
@@ -240,6 +339,72 @@ def run_batch(args):
                    "voxel_type": args.voxel_type, "kernels": "fused-plane batch" if fused else "generic",
                    "parallelism": "single GPU"},
         "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_learned(args):
+    """Learned correction: trains the network on (bf16 SART iterate 3, FP32
+    SART minimum-residual iterate within 50) pairs of seeded training phantoms,
+    then measures time to the target residual on a held-out phantom
+    (learned.time_to_target)."""
+    import torch
+    from ettention_amd import learned as L
+
+    wl = WORKLOADS["c5"]
+    n, N, k0, K, crop = wl["res"], wl["tilts"], wl["k0"], wl["k_max"], wl["crop"]
+    t0 = time.time()
+    pairs = []
+    for i in range(args.train_phantoms):
+        case, ph = L.make_case((n, n, n), N, seed=5001 + i)
+        del ph
+        early = L.sart_iterates(case, [k0], voxel_type="bfloat16", lam=wl["lam"], order=wl["order"])[0]
+        late, k_best, _ = L.sart_best_iterate(case, K, lam=wl["lam"], order=wl["order"])
+        pairs.append((early, late))
+        del case
+        log("training pair %d ready: late iterate %d (%.1f s)" % (i, k_best, time.time() - t0))
+    scale = 1.0
+    net = L.CorrectionNet((crop, crop, crop), seed=5000)
+    t1 = time.time()
+    losses = L.train_on_crops(net, pairs, crop, args.train_steps, scale, lr=1e-3, seed=1, log=log)
+    train_s = time.time() - t1
+    log("trained %d steps on %d^3 crops in %.1f s: loss %.4e -> %.4e"
+        % (args.train_steps, crop, train_s, losses[0], sum(losses[-20:]) / 20))
+    params = net.parameters()
+    del net, pairs
+    torch.cuda.empty_cache()
+    full = L.CorrectionNet((n, n, n), params)
+    case, ph = L.make_case((n, n, n), N, seed=5000)
+    del ph
+    x = torch.rand((n, n, n), device="cuda")
+    full.time_forward(x, scale)
+    fwd_ms, flops = full.time_forward(x, scale)
+    del x
+    log("network forward at %d^3: %.2f ms, %.1f TFLOP/s" % (n, fwd_ms, flops / fwd_ms / 1e9))
+    res = L.time_to_target(case, full, scale, k_max=K, k0=k0, lam=wl["lam"], order=wl["order"], log=log)
+    tflops = flops / fwd_ms / 1e9
+    out = {
+        "metric": "time-to-target-residual speedup, bf16 SART + learned correction vs plain FP32 SART",
+        "value": round(res["speedup"], 4) if res["speedup"] else 0.0,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * res["learned_s"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-storage/f32-accumulate (SART); bf16 MFMA, f32 accumulate (network)",
+        "data": "synthetic (seeded ellipsoid phantoms, FP projections + Poisson noise at 1e4 counts)",
+        "config": {"workload": wl["name"], "volume": [n, n, n], "projections": N, "k0": k0, "k_max": K,
+                   "train_phantoms": args.train_phantoms, "train_steps": args.train_steps, "train_crop": crop,
+                   "train_seconds": round(train_s, 2), "parallelism": "single GPU"},
+        "result": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res.items() if k != "plain_norms"},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 4), "traffic": None,
+                     "kernel": "conv3_fwd_kernel (whole network forward at %d^3)" % n,
+                     "avg_launch_ms": round(fwd_ms, 4), "flops_per_forward": flops},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

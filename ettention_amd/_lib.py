@@ -26,6 +26,7 @@ ETN_ALL_SUBVOLUMES = 0xFFFFFFFF
 ETN_RECON_FLAG_GENERIC_KERNELS = 1
 ETN_RECON_FLAG_NO_GRAPH = 2
 ETN_RECON_FLAG_NO_AUTOTUNE = 4
+ETN_RECON_FLAG_EXACT_ARITHMETIC = 8
 
 
 class EtnError(RuntimeError):

@@ -32,7 +32,8 @@ BatchReconstruction::BatchReconstruction(const etn_reconstruction_desc* members,
                 std::vector<FusedPlaneEngine::Member> cand;
                 for (uint32_t i : left) cand.push_back({ops_[i].get(), &members[i]});
                 std::vector<bool> taken;
-                std::unique_ptr<FusedPlaneEngine> e = FusedPlaneEngine::create(cand, stream_, &taken);
+                std::unique_ptr<FusedPlaneEngine> e =
+                    FusedPlaneEngine::create(cand, (flags & ETN_RECON_FLAG_EXACT_ARITHMETIC) != 0, stream_, &taken);
                 if (!e) break;
                 std::vector<uint32_t> rest;
                 size_t index = 0;

@@ -95,7 +95,8 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
 
     if ((d.flags & ETN_RECON_FLAG_GENERIC_KERNELS) == 0) {
         std::vector<bool> taken;
-        fused_ = FusedPlaneEngine::create({{this, &d}}, stream_, &taken);
+        fused_ = FusedPlaneEngine::create({{this, &d}}, (d.flags & ETN_RECON_FLAG_EXACT_ARITHMETIC) != 0, stream_,
+                                          &taken);
     }
     tuned_ = (d.flags & ETN_RECON_FLAG_NO_AUTOTUNE) != 0;
 }

@@ -119,7 +119,9 @@ struct PlaneRay {
 // band with unchanged state.  kCap keeps the reference's 2048-step limit; it is
 // dropped when nx + nz <= 2049, since every step moves vx or vz monotonically by
 // one, so a ray can take at most nx + nz - 1 steps inside the volume.
-template <bool kCap>
+// kFast accumulates dt * value with a fused multiply-add (one rounding instead
+// of two, as an OpenCL compiler may contract it).
+template <bool kCap, bool kFast>
 __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int width, int nx, int nz, int zlo,
                                               int zhi)
 {
@@ -145,7 +147,8 @@ __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, in
         // the sign of a zero, which changes neither the sums nor the next res values
         const bool ix = rx <= rz;
         const float dt = ix ? rx : rz;
-        acc += dt * band[li];
+        if (kFast) acc = fmaf(dt, band[li], acc);
+        else acc += dt * band[li];
         accl += dt;                                     // dt * 1.0f
         const float rest = (ix ? rz : rx) - dt;         // the axis not stepped loses dt
         rx = ix ? r.stepx : rest;
@@ -231,7 +234,15 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 
 // One workgroup per (member, plane group) work item.  `passes[m]` is member m's
 // pass of this launch; members without one get no work items.
-template <typename S, bool kVec, bool kCap, int RPT>
+//
+// kFast keeps every operation of the reference's algorithm but not its
+// rounding order: the bilinear lookup becomes A + B tx with A and B combined
+// per detector column at staging time (exactly the same function of tx and ty,
+// rounded differently), the weight chord / L uses the hardware reciprocal
+// (1 ulp; OpenCL allows 2.5 ulp for a division), and the ray sums use fused
+// multiply-adds.  Without kFast the kernel performs the generic kernels'
+// float operations one for one.
+template <typename S, bool kVec, bool kCap, bool kFast, int RPT>
 __global__ void __launch_bounds__(kFusedThreads)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
@@ -253,6 +264,7 @@ __global__ void __launch_bounds__(kFusedThreads)
     // per plane (m4 py, ty) of the current back projection
     float4* recs = reinterpret_cast<float4*>(lds + ((fv.band * width + 3) & ~3));
     float2* pterm = reinterpret_cast<float2*>(recs + np * 2 * row_len);
+    constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
     const size_t plane_stride = (size_t)nx * fv.ny;
     const size_t plane_base = (size_t)j * nx;
@@ -356,8 +368,14 @@ __global__ void __launch_bounds__(kFusedThreads)
                 v[4 + 2 * rr] = l0;
                 v[4 + 2 * rr + 1] = l1 - l0;
             }
-            recs[2 * e] = make_float4(v[0], v[1], v[2], v[3]);
-            recs[2 * e + 1] = make_float4(v[4], v[5], v[6], v[7]);
+            if (kFast) {
+                // R(tx) = A + B tx with A, B the row pair interpolated at ty
+                recs[e] = make_float4(fmaf(v[2] - v[0], ty, v[0]), fmaf(v[3] - v[1], ty, v[1]),
+                                      fmaf(v[6] - v[4], ty, v[4]), fmaf(v[7] - v[5], ty, v[5]));
+            } else {
+                recs[2 * e] = make_float4(v[0], v[1], v[2], v[3]);
+                recs[2 * e + 1] = make_float4(v[4], v[5], v[6], v[7]);
+            }
         }
         if (tid < np) {
             float py, ty;
@@ -435,6 +453,7 @@ __global__ void __launch_bounds__(kFusedThreads)
             }
             const ETN_GLOBAL FusedBp& bp = pass->bp[q];
             const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda, chord = bp.sample.w;
+            const float lc = lam * chord;
             // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
             auto x_term = [&](int x, float a4) {
                 const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + bp.sample.x;
@@ -451,6 +470,13 @@ __global__ void __launch_bounds__(kFusedThreads)
                 float bu;
                 const float tx = cl_fract(u - 0.5f, &bu);
                 const int i = padded_index((int)bu, nu);
+                if (kFast) {
+                    const float4 c = pr[i];
+                    const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
+                    const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
+                    value = (L > kEpsilon) ? value + contrib : value;
+                    return S::round(value);
+                }
                 const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
                 const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
                 const float L = L0 + (L1 - L0) * ty;
@@ -472,7 +498,7 @@ __global__ void __launch_bounds__(kFusedThreads)
                     x0 = xw - p * nx;
                 }
                 const float2 pt = pterm[p];
-                const float4* pr = recs + p * 2 * row_len;
+                const float4* pr = recs + p * kRecStride * row_len;
                 float xt[G];
 #pragma unroll
                 for (int i = 0; i < G; ++i) xt[i] = x_term(x0 + i, pt.x);
@@ -498,7 +524,7 @@ __global__ void __launch_bounds__(kFusedThreads)
                         x0 = xw - p * nx;
                     }
                     const float2 pt = pterm[p];
-                    const float4* pr = recs + p * 2 * row_len;
+                    const float4* pr = recs + p * kRecStride * row_len;
                     const float zt = z_term(zz);
                     float out[G];
 #pragma unroll
@@ -519,7 +545,7 @@ __global__ void __launch_bounds__(kFusedThreads)
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 const int id = k * kFusedThreads + tid;
-                if (id < nrays) plane_advance<kCap>(ray[k], band, width, nx, nz, z0, z1);
+                if (id < nrays) plane_advance<kCap, kFast>(ray[k], band, width, nx, nz, z0, z1);
             }
         }
         __syncthreads();
@@ -577,35 +603,42 @@ struct FusedLaunch {
     size_t lds;
 };
 
-template <typename S, bool kVec, bool kCap>
-void launch_fused_cap(const FusedLaunch& l, hipStream_t stream)
+template <typename S, bool kVec, bool kCap, bool kFast>
+void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 {
     const dim3 grid(l.groups), block(kFusedThreads);
     switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, 1><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, 2><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, 4><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, 8><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        case 1: fused_plane_kernel<S, kVec, kCap, kFast, 1><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, kFast, 2><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, kFast, 4><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, kFast, 8><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
     }
 }
 
-template <typename S>
+template <typename S, bool kFast>
 void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
-    if (vec && !cap) launch_fused_cap<S, true, false>(l, stream);
-    else if (vec) launch_fused_cap<S, true, true>(l, stream);
-    else if (!cap) launch_fused_cap<S, false, false>(l, stream);
-    else launch_fused_cap<S, false, true>(l, stream);
+    if (vec && !cap) launch_fused_rpt<S, true, false, kFast>(l, stream);
+    else if (vec) launch_fused_rpt<S, true, true, kFast>(l, stream);
+    else if (!cap) launch_fused_rpt<S, false, false, kFast>(l, stream);
+    else launch_fused_rpt<S, false, true, kFast>(l, stream);
 }
 
 }  // namespace
 
-// vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049
-void launch_fused(etn_voxel_type vt, bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
+// vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049;
+// exact: the generic kernels' operations and rounding order
+void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, const FusedLaunch& l, hipStream_t stream)
 {
-    if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32>(vec, cap, l, stream);
-    else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16>(vec, cap, l, stream);
-    else launch_fused_typed<BF16>(vec, cap, l, stream);
+    if (exact) {
+        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false>(vec, cap, l, stream);
+        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false>(vec, cap, l, stream);
+        else launch_fused_typed<BF16, false>(vec, cap, l, stream);
+    } else {
+        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true>(vec, cap, l, stream);
+        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true>(vec, cap, l, stream);
+        else launch_fused_typed<BF16, true>(vec, cap, l, stream);
+    }
     hip_check(hipGetLastError(), "fused plane kernel launch");
 }
 
@@ -867,7 +900,8 @@ void build_passes(MemberPlan& mb)
 
 class FusedEngineImpl final : public FusedPlaneEngine {
 public:
-    FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, hipStream_t stream);
+    FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
+                    hipStream_t stream);
     void enqueueIteration(hipStream_t stream) override;
     void check(hipStream_t stream) override;
     std::vector<LaunchTime> timeIteration(hipStream_t stream) override;
@@ -895,15 +929,15 @@ private:
 
     std::vector<MemberPlan> members_;
     etn_voxel_type vt_;
-    bool vec_, cap_;
+    bool vec_, cap_, exact_;
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
     Schedule sched_;
 };
 
-FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap,
+FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
                                  hipStream_t stream)
-    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap)
+    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact)
 {
     diag_.allocate(sizeof(int));
     ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
@@ -1042,7 +1076,7 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         const dev::FusedLaunch fl{l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
                                   tables + k * members_.size(), l.lds};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
-        dev::launch_fused(vt_, vec_, cap_, fl, stream);
+        dev::launch_fused(vt_, vec_, cap_, exact_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
     }
 }
@@ -1166,8 +1200,8 @@ void FusedEngineImpl::autotune(hipStream_t stream)
 
 }  // namespace
 
-std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Member>& members, hipStream_t stream,
-                                                           std::vector<bool>* taken)
+std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Member>& members, bool exact,
+                                                           hipStream_t stream, std::vector<bool>* taken)
 {
     taken->assign(members.size(), false);
     std::vector<MemberPlan> plans;
@@ -1188,7 +1222,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
         (*taken)[i] = true;
     }
     if (plans.empty()) return nullptr;
-    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, stream);
+    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, exact, stream);
 }
 
 }  // namespace etn

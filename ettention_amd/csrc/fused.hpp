@@ -32,9 +32,11 @@ public:
     // samples, subvolumes and mask inside what the fused kernels cover) and
     // share the first such member's voxel type and row-vector width.
     // taken[i] tells whether members[i] is driven by the engine; null when
-    // none is.
-    static std::unique_ptr<FusedPlaneEngine> create(const std::vector<Member>& members, hipStream_t stream,
-                                                    std::vector<bool>* taken);
+    // none is.  exact: the generic kernels' float operations in their order
+    // (bit-identical volumes); otherwise the reference's operations with
+    // combined bilinear weights, a hardware reciprocal and fused multiply-adds.
+    static std::unique_ptr<FusedPlaneEngine> create(const std::vector<Member>& members, bool exact,
+                                                    hipStream_t stream, std::vector<bool>* taken);
     virtual ~FusedPlaneEngine() = default;
     virtual void enqueueIteration(hipStream_t stream) = 0;
     // Throws when a kernel found a live ray outside the plane its row was assigned to.

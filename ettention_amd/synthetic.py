@@ -169,8 +169,15 @@ def add_gaussian_noise(projections, sigma_rel, seed):
 def add_poisson_noise(projections, counts, seed, peak_attenuation=2.0):
     """Transmission noise: line integrals are scaled so the largest is
     `peak_attenuation`, counts ~ Poisson(counts * exp(-b)), b' = -ln(max(n,1)/counts),
-    then scaled back.  numpy in, numpy out."""
-    rng = np.random.default_rng(seed)
+    then scaled back.  numpy in, numpy out; a torch tensor is noised on its own
+    device with a torch generator seeded `seed`, and returned as float32."""
     scale = peak_attenuation / max(float(projections.max()), 1e-12)
+    if not isinstance(projections, np.ndarray):
+        import torch
+        g = torch.Generator(device=projections.device).manual_seed(seed)
+        rate = counts * torch.exp(-scale * projections.double())
+        n = torch.poisson(rate, generator=g)
+        return (-torch.log(torch.clamp(n, min=1.0) / counts) / scale).float()
+    rng = np.random.default_rng(seed)
     n = rng.poisson(counts * np.exp(-scale * projections.astype(np.float64)))
     return (-np.log(np.maximum(n, 1) / counts) / scale).astype(np.float32)

@@ -159,7 +159,11 @@ typedef struct etn_reconstruction etn_reconstruction;
 enum {
     ETN_RECON_FLAG_GENERIC_KERNELS = 1u << 0,   /* disable the fused plane kernels */
     ETN_RECON_FLAG_NO_GRAPH = 1u << 1,          /* launch eagerly instead of replaying a hipGraph */
-    ETN_RECON_FLAG_NO_AUTOTUNE = 1u << 2        /* keep the default fused tile instead of timing candidates */
+    ETN_RECON_FLAG_NO_AUTOTUNE = 1u << 2,       /* keep the default fused tile instead of timing candidates */
+    ETN_RECON_FLAG_EXACT_ARITHMETIC = 1u << 3   /* fused kernels repeat the generic kernels' float operations in
+                                                   their order (bit-identical volumes); by default they combine
+                                                   the bilinear weights per detector column, use the hardware
+                                                   reciprocal for chord / L and fused multiply-adds */
 };
 
 /* Parameters of BlockIterativeReconstructionOperator, named after the

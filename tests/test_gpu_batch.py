@@ -1,11 +1,12 @@
 """Batched reconstructions and the fused kernels' tiles.
 
 A batch advances independent reconstructions of different sizes together,
-each fused launch carrying one pass of every member; a member's volume must
-be bit-identical to the same reconstruction run alone with the generic
-kernels.  The tile (planes per workgroup, z rows per LDS band) only changes
-how the work is cut, so every tile, and whatever the autotuner picks, must
-give bit-identical volumes too."""
+each fused launch carrying one pass of every member; with exact arithmetic a
+member's volume must be bit-identical to the same reconstruction run alone
+with the generic kernels, and with the default arithmetic bit-identical to it
+run alone on the fused path.  The tile (planes per workgroup, z rows per LDS
+band) only changes how the work is cut, so every tile, and whatever the
+autotuner picks, must give bit-identical volumes too."""
 import numpy as np
 import pytest
 
@@ -59,8 +60,11 @@ def members():
     ]
 
 
+EXACT = E.ETN_RECON_FLAG_EXACT_ARITHMETIC
+
+
 def test_batch_members_equal_standalone_generic(members):
-    b = A.BatchReconstruction(members)
+    b = A.BatchReconstruction(members, flags=EXACT)
     fused = [b.uses_fused_path(i) for i in range(len(b))]
     assert fused == [True, True, True, True, True, True, False, False]
     b.run_iterations(2)
@@ -75,6 +79,16 @@ def test_batch_members_equal_standalone_generic(members):
             assert planes in (1, 2, 4, 8) and m["volume"].resolution[1] % planes == 0 and band >= 1
     ms, n = b.time_iteration()
     assert ms > 0.0 and n >= max(len(m["tilt_angles"]) for m in members[:6])
+
+
+def test_batch_members_equal_standalone_fused(members):
+    # default arithmetic: each member as it runs alone on the fused path
+    sel = members[:4]
+    b = A.BatchReconstruction(sel)
+    b.run_iterations(2)
+    for i, m in enumerate(sel):
+        _, ref = alone(m, 2, flags=0)
+        np.testing.assert_array_equal(b.get_reconstructed_volume(i, dtype=torch.float32).cpu().numpy(), ref)
 
 
 def test_batch_graph_replays_equal_eager(members):
@@ -104,7 +118,7 @@ def test_batch_all_generic_flag(members):
 def test_every_tile_gives_the_same_volume(members, tile):
     m = members[0]   # 64^3, 41 tilts
     _, ref = alone(m, 2)
-    r = A.BlockIterativeReconstructionOperator(**m)
+    r = A.BlockIterativeReconstructionOperator(flags=EXACT, **m)
     r.set_tile(*tile)
     assert r.tile() == tile
     r.run_iterations(2)
@@ -125,7 +139,7 @@ def test_invalid_tiles_are_rejected(members):
 def test_autotuned_equals_default_tile(members):
     m = members[1]
     outs = []
-    for flags in (0, E.ETN_RECON_FLAG_NO_AUTOTUNE):
+    for flags in (EXACT, EXACT | E.ETN_RECON_FLAG_NO_AUTOTUNE):
         r = A.BlockIterativeReconstructionOperator(flags=flags, **m)
         r.run_single_iteration()     # the autotuner runs first and restores the volume
         r.run_single_iteration()
