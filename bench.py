@@ -64,8 +64,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-projections", type=int, default=32,
                     help="projection steps the CPU baseline times (about 0.35 s each on 16 threads)")
-    ap.add_argument("--train-phantoms", type=int, default=6, help="c5: training phantoms (512^3 each)")
-    ap.add_argument("--train-steps", type=int, default=600, help="c5: Adam steps on 128^3 crops")
+    ap.add_argument("--train-phantoms", type=int, default=8, help="c5: training phantoms (512^3 each)")
+    ap.add_argument("--train-steps", type=int, default=2000, help="c5: Adam steps on 128^3 crops")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (traffic calibration for rocprof counters)")
     args = ap.parse_args()
@@ -345,34 +345,33 @@ def run_batch(args):
 
 
 def run_learned(args):
-    """Learned correction: trains the network on (bf16 SART iterate 3, FP32
-    SART minimum-residual iterate within 50) pairs of seeded training phantoms,
-    then measures time to the target residual on a held-out phantom
-    (learned.time_to_target)."""
+    """Learned correction: trains the network on samples of seeded training
+    phantoms (input: bf16 SART iterate 3 and its last update; target: the FP32
+    SART minimum-residual iterate within 50), then measures time to the target
+    residual on a held-out phantom (learned.time_to_target)."""
     import torch
     from ettention_amd import learned as L
 
     wl = WORKLOADS["c5"]
     n, N, k0, K, crop = wl["res"], wl["tilts"], wl["k0"], wl["k_max"], wl["crop"]
     t0 = time.time()
-    pairs = []
+    samples = []
     for i in range(args.train_phantoms):
         case, ph = L.make_case((n, n, n), N, seed=5001 + i)
         del ph
-        early = L.sart_iterates(case, [k0], voxel_type="bfloat16", lam=wl["lam"], order=wl["order"])[0]
-        late, k_best, _ = L.sart_best_iterate(case, K, lam=wl["lam"], order=wl["order"])
-        pairs.append((early, late))
+        sample, k_best = L.training_sample(case, k0, K, lam=wl["lam"], order=wl["order"])
+        samples.append(sample)
         del case
-        log("training pair %d ready: late iterate %d (%.1f s)" % (i, k_best, time.time() - t0))
+        log("training sample %d ready: late iterate %d (%.1f s)" % (i, k_best, time.time() - t0))
     scale = 1.0
     net = L.CorrectionNet((crop, crop, crop), seed=5000)
     t1 = time.time()
-    losses = L.train_on_crops(net, pairs, crop, args.train_steps, scale, lr=1e-3, seed=1, log=log)
+    losses = L.train_on_crops(net, samples, crop, args.train_steps, scale, lr=1e-3, seed=1, log=log)
     train_s = time.time() - t1
-    log("trained %d steps on %d^3 crops in %.1f s: loss %.4e -> %.4e"
-        % (args.train_steps, crop, train_s, losses[0], sum(losses[-20:]) / 20))
+    log("trained %d steps on %d^3 crops in %.1f s: mean loss of the first / last 100 steps %.4e -> %.4e"
+        % (args.train_steps, crop, train_s, np.mean(losses[:100]), np.mean(losses[-100:])))
     params = net.parameters()
-    del net, pairs
+    del net, samples
     torch.cuda.empty_cache()
     full = L.CorrectionNet((n, n, n), params)
     case, ph = L.make_case((n, n, n), N, seed=5000)

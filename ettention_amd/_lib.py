@@ -145,9 +145,9 @@ SIGNATURES = [
     ("etn_unet_parameter_count", _i32, [C.POINTER(C.c_size_t)]),
     ("etn_unet_set_parameters", _i32, [_vp, _vp, C.c_size_t]),
     ("etn_unet_get_parameters", _i32, [_vp, _vp, C.c_size_t]),
-    ("etn_unet_predict", _i32, [_vp, _vp, _f32, _vp]),
-    ("etn_unet_gradient", _i32, [_vp, _vp, _vp, _f32, _vp, C.POINTER(C.c_double)]),
-    ("etn_unet_train_step", _i32, [_vp, _vp, _vp, _f32, _f32, C.POINTER(C.c_double)]),
+    ("etn_unet_predict", _i32, [_vp, _vp, _vp, _f32, _f32, _vp]),
+    ("etn_unet_gradient", _i32, [_vp, _vp, _vp, _vp, _f32, _f32, _vp, C.POINTER(C.c_double)]),
+    ("etn_unet_train_step", _i32, [_vp, _vp, _vp, _vp, _f32, _f32, _f32, C.POINTER(C.c_double)]),
     ("etn_unet_time_forward", _i32, [_vp, _vp, _f32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 

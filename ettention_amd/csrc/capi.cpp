@@ -516,37 +516,38 @@ etn_status etn_unet_get_parameters(etn_unet* n, float* host, size_t count)
     });
 }
 
-etn_status etn_unet_predict(etn_unet* n, const float* x, float scale, float* out)
+etn_status etn_unet_predict(etn_unet* n, const float* x, const float* d, float scale, float update_scale,
+                            float* out)
 {
     return guarded([&] {
         need(n, "network");
         need(x, "input volume");
         need(out, "output volume");
-        n->net->predict(x, scale, out);
+        n->net->predict(x, d, scale, update_scale, out);
     });
 }
 
-etn_status etn_unet_gradient(etn_unet* n, const float* x, const float* target, float scale, float* grad_host,
-                             double* loss)
+etn_status etn_unet_gradient(etn_unet* n, const float* x, const float* d, const float* target, float scale,
+                             float update_scale, float* grad_host, double* loss)
 {
     return guarded([&] {
         need(n, "network");
         need(x, "input volume");
         need(target, "target volume");
-        const double l = n->net->gradient(x, target, scale);
+        const double l = n->net->gradient(x, d, target, scale, update_scale);
         if (loss) *loss = l;
         if (grad_host) n->net->copyGradient(grad_host, etn::UNet3D::parameterCount());
     });
 }
 
-etn_status etn_unet_train_step(etn_unet* n, const float* x, const float* target, float scale, float lr,
-                               double* loss)
+etn_status etn_unet_train_step(etn_unet* n, const float* x, const float* d, const float* target, float scale,
+                               float update_scale, float lr, double* loss)
 {
     return guarded([&] {
         need(n, "network");
         need(x, "input volume");
         need(target, "target volume");
-        const double l = n->net->trainStep(x, target, scale, lr);
+        const double l = n->net->trainStep(x, d, target, scale, update_scale, lr);
         if (loss) *loss = l;
     });
 }

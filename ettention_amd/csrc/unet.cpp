@@ -12,9 +12,9 @@ struct LayerSpec {
     int cin, cout, level;
 };
 
-// c4a reads concat(upsample(c3b), c2b), c5a reads concat(upsample(c4b), c1b)
+// c1a reads (x, d); c4a reads concat(upsample(c3b), c2b), c5a reads concat(upsample(c4b), c1b)
 constexpr LayerSpec kLayers[] = {
-    {"c1a", 1, 16, 0},  {"c1b", 16, 16, 0}, {"c2a", 16, 32, 1}, {"c2b", 32, 32, 1}, {"c3a", 32, 64, 2},
+    {"c1a", 2, 16, 0},  {"c1b", 16, 16, 0}, {"c2a", 16, 32, 1}, {"c2b", 32, 32, 1}, {"c3a", 32, 64, 2},
     {"c3b", 64, 64, 2}, {"c4a", 96, 32, 1}, {"c4b", 32, 32, 1}, {"c5a", 48, 16, 0}, {"c5b", 16, 16, 0},
 };
 constexpr int kLayerCount = 10;
@@ -50,10 +50,10 @@ UNet3D::UNet3D(int nx, int ny, int nz, hipStream_t stream) : stream_(stream)
     ETN_HIP(hipMemsetAsync(params_.get(), 0, sizeof(float) * P, stream_));
     for (const Layer& L : layers_) {
         wt_.emplace_back(sizeof(uint16_t) * L.cout * nn::kpad_for(L.cin));
-        wb_.emplace_back(L.cin > 1 ? sizeof(uint16_t) * L.cin * nn::kpad_for(L.cout) : 0);
+        wb_.emplace_back(L.cin >= 8 ? sizeof(uint16_t) * L.cin * nn::kpad_for(L.cout) : 0);
         act_.emplace_back(sizeof(uint16_t) * voxels(L.level) * L.cout);
     }
-    xin_.allocate(sizeof(uint16_t) * voxels(0));
+    xin_.allocate(sizeof(uint16_t) * voxels(0) * 2);
     pool1_.allocate(sizeof(uint16_t) * voxels(1) * 16);
     pool2_.allocate(sizeof(uint16_t) * voxels(2) * 32);
     pred_.allocate(sizeof(float) * voxels(0));
@@ -85,7 +85,7 @@ void UNet3D::packAll()
     for (int i = 0; i < kLayerCount; ++i) {
         const Layer& L = layers_[i];
         nn::pack_weights(params_.as<float>() + L.w, L.cout, L.cin, wt_[i].as<uint16_t>(),
-                         L.cin > 1 ? wb_[i].as<uint16_t>() : nullptr, stream_);
+                         L.cin >= 8 ? wb_[i].as<uint16_t>() : nullptr, stream_);
     }
 }
 
@@ -93,7 +93,7 @@ nn::ConvSrc UNet3D::layerInput(int i) const
 {
     nn::ConvSrc s;
     switch (i) {
-        case 0: s.a = xin_.as<uint16_t>(); s.ca = 1; break;
+        case 0: s.a = xin_.as<uint16_t>(); s.ca = 2; break;
         case 2: s.a = pool1_.as<uint16_t>(); s.ca = 16; break;
         case 4: s.a = pool2_.as<uint16_t>(); s.ca = 32; break;
         case 6:
@@ -109,9 +109,9 @@ nn::ConvSrc UNet3D::layerInput(int i) const
     return s;
 }
 
-void UNet3D::forward(const float* x, float scale)
+void UNet3D::forward(const float* x, const float* d, float scale, float dscale)
 {
-    nn::to_bf16_scaled(x, scale, voxels(0), xin_.as<uint16_t>(), stream_);
+    nn::pack_input(x, scale, d, dscale, voxels(0), xin_.as<uint16_t>(), stream_);
     for (int i = 0; i < kLayerCount; ++i) {
         const Layer& L = layers_[i];
         if (i == 2) nn::pool2(layerOutput(1), D_[0], H_[0], W_[0], 16, pool1_.as<uint16_t>(), stream_);
@@ -133,10 +133,10 @@ void UNet3D::forward(const float* x, float scale)
     nn::head_forward(layerOutput(9), x, scale, p + head_, p + head_ + kHead, voxels(0), pred_.as<float>(), stream_);
 }
 
-void UNet3D::predict(const float* x, float scale, float* out)
+void UNet3D::predict(const float* x, const float* d, float scale, float dscale, float* out)
 {
-    if (!(scale > 0.0f) || !std::isfinite(scale)) fail("scale must be positive and finite");
-    forward(x, scale);
+    if (!(scale > 0.0f) || !std::isfinite(scale) || !std::isfinite(dscale)) fail("scales must be finite, scale > 0");
+    forward(x, d, scale, dscale);
     nn::unscale_prediction(pred_.as<float>(), x, scale, voxels(0), out, stream_);
 }
 
@@ -228,11 +228,11 @@ void UNet3D::backward()
     layer_back(0, gtmp, nullptr);
 }
 
-double UNet3D::gradient(const float* x, const float* target, float scale)
+double UNet3D::gradient(const float* x, const float* d, const float* target, float scale, float dscale)
 {
-    if (!(scale > 0.0f) || !std::isfinite(scale)) fail("scale must be positive and finite");
+    if (!(scale > 0.0f) || !std::isfinite(scale) || !std::isfinite(dscale)) fail("scales must be finite, scale > 0");
     ensureTraining();
-    forward(x, scale);
+    forward(x, d, scale, dscale);
     nn::mse_loss(pred_.as<float>(), target, scale, voxels(0), gpred_.as<float>(), lpart_.as<double>(),
                  loss_.as<double>(), stream_);
     backward();
@@ -250,9 +250,9 @@ void UNet3D::copyGradient(float* host, size_t count) const
     ETN_HIP(hipStreamSynchronize(stream_));
 }
 
-double UNet3D::trainStep(const float* x, const float* target, float scale, float lr)
+double UNet3D::trainStep(const float* x, const float* d, const float* target, float scale, float dscale, float lr)
 {
-    const double loss = gradient(x, target, scale);
+    const double loss = gradient(x, d, target, scale, dscale);
     ++adam_t_;
     const float b1 = 0.9f, b2 = 0.999f;
     const float bc1 = 1.0f - std::pow(b1, (float)adam_t_), bc2 = 1.0f - std::pow(b2, (float)adam_t_);
@@ -272,7 +272,7 @@ double UNet3D::timeForward(const float* x, float scale, double* flops)
     ETN_HIP(hipEventCreate(&a));
     ETN_HIP(hipEventCreate(&b));
     ETN_HIP(hipEventRecord(a, stream_));
-    forward(x, scale);
+    forward(x, nullptr, scale, 0.0f);
     ETN_HIP(hipEventRecord(b, stream_));
     ETN_HIP(hipEventSynchronize(b));
     float ms = 0.0f;

@@ -1,7 +1,8 @@
 // Learned correction network: a 3-level 3-D U-Net (16/32/64 channels, 3x3x3
 // convolutions with ReLU, 2x2x2 average pooling, nearest upsampling with
-// channel-concatenated skips, a 1x1x1 head) that maps an early SART iterate to
-// a late one as x + delta(x).  Weights and activations are bf16 on the
+// channel-concatenated skips, a 1x1x1 head) that maps an early SART iterate x
+// and its last update d (x minus the iterate before it) to a late iterate as
+// x + delta(x, d).  Weights and activations are bf16 on the
 // matrix cores with FP32 accumulation; master weights, gradients and the Adam
 // state are FP32.  Parameter order (flat vector): per layer c1a c1b c2a c2b
 // c3a c3b c4a c4b c5a c5b the weights [cout][27][cin] then the bias [cout];
@@ -32,18 +33,18 @@ public:
     void setParameters(const float* host, size_t count);
     void getParameters(float* host, size_t count) const;
 
-    // out = x + delta(x * scale) / scale (device fp32 volumes)
-    void predict(const float* x, float scale, float* out);
+    // out = x + delta(x scale, d dscale) / scale (device fp32 volumes; d may be null: zeros)
+    void predict(const float* x, const float* d, float scale, float dscale, float* out);
     // forward + backward of L = mean((x s + delta - target s)^2); gradients stay on the device
-    double gradient(const float* x, const float* target, float scale);
+    double gradient(const float* x, const float* d, const float* target, float scale, float dscale);
     void copyGradient(float* host, size_t count) const;
     // gradient + one Adam step (beta1 0.9, beta2 0.999, eps 1e-8)
-    double trainStep(const float* x, const float* target, float scale, float lr);
+    double trainStep(const float* x, const float* d, const float* target, float scale, float dscale, float lr);
     // HIP-event time of one forward pass; flops = 2 x multiply-adds of its convolutions
     double timeForward(const float* x, float scale, double* flops);
 
 private:
-    void forward(const float* x, float scale);
+    void forward(const float* x, const float* d, float scale, float dscale);
     void backward();
     void packAll();
     void ensureTraining();

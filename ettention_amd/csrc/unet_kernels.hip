@@ -40,7 +40,9 @@ constexpr int kConvWaves = 4;
 constexpr int kThreads = 256;
 
 // ---- 3x3x3 convolution, implicit GEMM ---------------------------------------
-template <int NT, int MT, bool kCin1>
+// kSmall: the network input, ca < 8 channels and no b; k = tap * ca + ci
+// gathers single values (the 27 ca products are padded to a multiple of 32)
+template <int NT, int MT, bool kSmall>
 __global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -70,7 +72,7 @@ __global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    const int ca = a.src.ca, cin = kCin1 ? 1 : a.src.ca + a.src.cb;
+    const int ca = a.src.ca, cin = a.src.ca + a.src.cb;
     const int Hh = a.H >> 1, Wh = a.W >> 1;
     const bf16x8 zero = as_bf16x8(u32x4{0u, 0u, 0u, 0u});
     for (int k0 = kg * 8; k0 < a.kpad; k0 += 32) {
@@ -78,18 +80,18 @@ __global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) B[nt] = as_bf16x8(load16(a.wt + (size_t)(nt * 16 + row) * a.kpad + k0));
         bf16x8 A[MT];
-        if (kCin1) {
+        if (kSmall) {
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 uint16_t v[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const int t = k0 + e;
+                    const int t = (k0 + e) / ca, ci = k0 + e - t * ca;
                     v[e] = 0;
                     if (t < 27) {
                         const int zz = vz[mt] + t / 9 - 1, yy = vy[mt] + (t / 3) % 3 - 1, xx = vx[mt] + t % 3 - 1;
                         if ((unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W)
-                            v[e] = a.src.a[((size_t)zz * a.H + yy) * a.W + xx];
+                            v[e] = a.src.a[(((size_t)zz * a.H + yy) * a.W + xx) * ca + ci];
                     }
                 }
                 A[mt] = as_bf16x8(u32x4{v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16),
@@ -146,7 +148,7 @@ void launch_conv(const ConvArgs& a, hipStream_t stream)
     const size_t M = (size_t)a.D * a.H * a.W;
     const size_t per_block = (size_t)kConvWaves * MT * 16;
     const dim3 grid((unsigned)((M + per_block - 1) / per_block));
-    if (a.src.ca == 1 && a.src.cb == 0)
+    if (a.src.ca < 8 && a.src.cb == 0)
         conv3_fwd_kernel<NT, MT, true><<<grid, kThreads, 0, stream>>>(a);
     else
         conv3_fwd_kernel<NT, MT, false><<<grid, kThreads, 0, stream>>>(a);
@@ -166,7 +168,7 @@ struct WgradArgs {
     float* partial;     // [chunks][27][cout][cin16]
 };
 
-template <bool kCin1>
+template <bool kSmall>
 __global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
 {
     __shared__ __attribute__((aligned(16))) uint16_t gyT[kMaxCout * kLdsRow];
@@ -177,12 +179,12 @@ __global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
     const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
     const size_t M = (size_t)w.D * w.H * w.W;
     const int HW = w.H * w.W;
-    const int ca = w.src.ca, cin = kCin1 ? 1 : w.src.ca + w.src.cb;
+    const int ca = w.src.ca, cin = w.src.ca + w.src.cb;
     const int cin16 = (cin + 15) / 16 * 16;
     const int nco = w.cout / 16, nci = cin16 / 16, ntiles = nco * nci;
     const int Hh = w.H >> 1, Wh = w.W >> 1;
     const size_t mbeg = (size_t)blockIdx.x * w.chunk;
-    if (kCin1) {
+    if (kSmall) {   // rows ca..15 of xT stay zero
         for (int i = threadIdx.x; i < 16 * kLdsRow; i += kThreads) xT[i] = 0;
     }
     floatx4 acc[4];
@@ -205,17 +207,18 @@ __global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
             }
         }
         // tap-shifted input, transposed: xT[ci][voxel]
-        if (kCin1) {
-            for (int mi = threadIdx.x; mi < 32; mi += kThreads) {
+        if (kSmall) {
+            for (int g = threadIdx.x; g < 32 * ca; g += kThreads) {
+                const int mi = g / ca, ci = g - mi * ca;
                 const size_t m = ms + mi;
                 uint16_t v = 0;
                 if (m < M) {
                     const int z = (int)(m / HW), r = (int)(m - (size_t)z * HW), y = r / w.W, x = r - y * w.W;
                     const int zz = z + dz, yy = y + dy, xx = x + dx;
                     if ((unsigned)zz < (unsigned)w.D && (unsigned)yy < (unsigned)w.H && (unsigned)xx < (unsigned)w.W)
-                        v = w.src.a[((size_t)zz * w.H + yy) * w.W + xx];
+                        v = w.src.a[(((size_t)zz * w.H + yy) * w.W + xx) * ca + ci];
                 }
-                xT[mi] = v;
+                xT[ci * kLdsRow + mi] = v;
             }
         } else {
             const int gpx = cin / 8;
@@ -391,10 +394,12 @@ __global__ void take_channels_kernel(const uint16_t* g, int stride, int c0, size
     }
 }
 
-__global__ void to_bf16_scaled_kernel(const float* x, float scale, size_t n, uint16_t* y)
+__global__ void pack_input_kernel(const float* x, float scale, const float* d, float dscale, size_t n, uint16_t* y)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        y[i] = f2bf(x[i] * scale);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        y[2 * i] = f2bf(x[i] * scale);
+        y[2 * i + 1] = d ? f2bf(d[i] * dscale) : (uint16_t)0;
+    }
 }
 
 __global__ void head_forward_kernel(const uint16_t* d1, const float* x, float scale, const float* hw, const float* hb,
@@ -497,9 +502,10 @@ unsigned grid_for(size_t n)
 
 void conv3_forward(const ConvArgs& a, hipStream_t stream)
 {
-    const bool cin1 = a.src.ca == 1 && a.src.cb == 0;
-    if (!cin1 && ((a.src.ca % 8) || (a.src.cb % 8))) fail("conv3: input channels must be multiples of 8");
-    if (a.kpad != kpad_for(cin1 ? 1 : a.src.ca + a.src.cb)) fail("conv3: weight row length mismatch");
+    const bool small = a.src.ca < 8 && a.src.cb == 0;
+    if (a.src.ca < 1 || (!small && ((a.src.ca % 8) || (a.src.cb % 8))))
+        fail("conv3: input channels must be multiples of 8 (or fewer than 8 without a second source)");
+    if (a.kpad != kpad_for(a.src.ca + a.src.cb)) fail("conv3: weight row length mismatch");
     if (a.src.a_half && ((a.D | a.H | a.W) & 1)) fail("conv3: upsampled input needs even sizes");
     switch (a.cout) {
         case 16: launch_conv<1, 4>(a, stream); break;
@@ -515,12 +521,12 @@ void conv3_forward(const ConvArgs& a, hipStream_t stream)
 void conv3_weight_grad(const ConvSrc& src, int D, int H, int W, const uint16_t* gy, int cout, float* partial,
                        hipStream_t stream)
 {
-    const bool cin1 = src.ca == 1 && src.cb == 0;
+    const bool small = src.ca < 8 && src.cb == 0;
     if (cout % 16 || cout > kMaxCout || src.ca + src.cb > kMaxCin) fail("conv3 weight gradient: channel counts");
     WgradArgs w{src, D, H, W, gy, cout, kWgradChunk, partial};
     const size_t M = (size_t)D * H * W;
     const dim3 grid((unsigned)wgrad_chunks(M), 27);
-    if (cin1) conv3_wgrad_kernel<true><<<grid, kThreads, 0, stream>>>(w);
+    if (small) conv3_wgrad_kernel<true><<<grid, kThreads, 0, stream>>>(w);
     else conv3_wgrad_kernel<false><<<grid, kThreads, 0, stream>>>(w);
     hip_check(hipGetLastError(), "conv3 weight gradient launch");
 }
@@ -580,9 +586,10 @@ void take_channels(const uint16_t* g, int stride, int c0, size_t voxels, int C, 
     take_channels_kernel<<<grid_for(voxels * C), kThreads, 0, stream>>>(g, stride, c0, voxels, C, dst, accumulate);
 }
 
-void to_bf16_scaled(const float* x, float scale, size_t count, uint16_t* y, hipStream_t stream)
+void pack_input(const float* x, float scale, const float* d, float dscale, size_t count, uint16_t* y,
+                hipStream_t stream)
 {
-    to_bf16_scaled_kernel<<<grid_for(count), kThreads, 0, stream>>>(x, scale, count, y);
+    pack_input_kernel<<<grid_for(count), kThreads, 0, stream>>>(x, scale, d, dscale, count, y);
 }
 
 void head_forward(const uint16_t* d1, const float* x, float scale, const float* hw, const float* hb, size_t voxels,

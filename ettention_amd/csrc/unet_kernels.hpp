@@ -15,7 +15,7 @@ namespace nn {
 // Input of a convolution: channels [0, ca) from `a` (at half resolution,
 // read through nearest-neighbour upsampling, when a_half), then channels
 // [ca, ca + cb) from `b` at full resolution.  ca and cb are multiples of 8,
-// or ca = 1 with no b (the network input).
+// or ca < 8 with no b (the network input).
 struct ConvSrc {
     const uint16_t* a = nullptr;
     int ca = 0;
@@ -68,7 +68,9 @@ void up2_backward(const uint16_t* g, int stride, int c0, int D, int H, int W, in
 void take_channels(const uint16_t* g, int stride, int c0, size_t voxels, int C, uint16_t* dst, int accumulate,
                    hipStream_t stream);
 
-void to_bf16_scaled(const float* x, float scale, size_t count, uint16_t* y, hipStream_t stream);
+// y[m] = (x[m] scale, d[m] dscale) as two bf16 channels; d may be null (zeros)
+void pack_input(const float* x, float scale, const float* d, float dscale, size_t count, uint16_t* y,
+                hipStream_t stream);
 // pred[m] = x[m] * scale + (sum_c hw[c] d1[m][c] + hb), fp32
 void head_forward(const uint16_t* d1, const float* x, float scale, const float* hw, const float* hb, size_t voxels,
                   float* pred, hipStream_t stream);

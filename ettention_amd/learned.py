@@ -2,12 +2,18 @@
 
 A 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions, bf16 on the
 matrix cores, libetn.so's own forward and backward kernels) is trained on
-pairs (SART iterate k0, SART iterate k1) that the reconstruction itself
-produces, and predicts iterate k1 from iterate k0 as x + delta(x).
+samples that the reconstruction itself produces: the SART iterate x_k0, its
+last update d = x_k0 - x_(k0-1), and a late iterate as the target.  It
+predicts the late iterate as x_k0 + delta(x_k0, d).
 
-Plain SART stays the judge: `LearnedSart.run` returns the corrected volume
-only if its projection residual ||Ax - b|| is below what plain SART reaches
-in the same wall-clock time, and plain SART's volume otherwise.
+SART as the reference implements it semi-converges: ||Ax - b|| falls for the
+first iterations and then grows (the voxel-driven bilinear back projection,
+with its half-pixel offset, is not the forward projector's adjoint), on
+noiseless data as well.  The late iterate is therefore the one with the
+lowest residual within k_max iterations, not iterate k_max itself.
+
+Plain SART stays the judge: a jump is accepted only if its projection
+residual is below what plain SART reaches in the same wall-clock time.
 """
 from __future__ import annotations
 
@@ -18,6 +24,10 @@ import numpy as np
 
 from . import api as A
 from ._lib import call, lib
+
+# the network sees (x scale, d scale UPDATE_GAIN): updates are about an
+# order of magnitude smaller than the iterate
+UPDATE_GAIN = 16.0
 
 
 def _torch():
@@ -31,9 +41,9 @@ def parameter_count():
     return n.value
 
 
-# (name, input channels, output channels) in parameter order; c4a / c5a read
-# concat(upsample(lower level), skip)
-LAYERS = [("c1a", 1, 16), ("c1b", 16, 16), ("c2a", 16, 32), ("c2b", 32, 32), ("c3a", 32, 64),
+# (name, input channels, output channels) in parameter order; c1a reads (x, d),
+# c4a / c5a read concat(upsample(lower level), skip)
+LAYERS = [("c1a", 2, 16), ("c1b", 16, 16), ("c2a", 16, 32), ("c2b", 32, 32), ("c3a", 32, 64),
           ("c3b", 64, 64), ("c4a", 96, 32), ("c4b", 32, 32), ("c5a", 48, 16), ("c5b", 16, 16)]
 
 
@@ -51,8 +61,13 @@ def init_parameters(seed):
     return out
 
 
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
 class CorrectionNet:
-    """etn_unet_*: the network at a fixed working size (volumes [nz][ny][nx])."""
+    """etn_unet_*: the network at a fixed working size (volumes [nz][ny][nx],
+    float32 device tensors)."""
 
     def __init__(self, shape_zyx, params=None, seed=5000, stream=None):
         nz, ny, nx = shape_zyx
@@ -82,91 +97,33 @@ class CorrectionNet:
         call("etn_unet_get_parameters", self._h, p.ctypes.data_as(C.c_void_p), p.size)
         return p
 
-    def predict(self, x, scale, out=None):
+    def predict(self, x, d, scale, update_scale=None, out=None):
+        """x + delta(x scale, d update_scale) / scale; d may be None (zeros)."""
         torch = _torch()
+        us = scale * UPDATE_GAIN if update_scale is None else update_scale
         out = torch.empty_like(x) if out is None else out
-        call("etn_unet_predict", self._h, C.c_void_p(x.data_ptr()), float(scale), C.c_void_p(out.data_ptr()))
+        call("etn_unet_predict", self._h, _p(x), _p(d), float(scale), float(us), _p(out))
         return out
 
-    def gradient(self, x, target, scale):
+    def gradient(self, x, d, target, scale, update_scale=None):
+        us = scale * UPDATE_GAIN if update_scale is None else update_scale
         g = np.zeros(parameter_count(), np.float32)
         loss = C.c_double()
-        call("etn_unet_gradient", self._h, C.c_void_p(x.data_ptr()), C.c_void_p(target.data_ptr()), float(scale),
+        call("etn_unet_gradient", self._h, _p(x), _p(d), _p(target), float(scale), float(us),
              g.ctypes.data_as(C.c_void_p), C.byref(loss))
         return loss.value, g
 
-    def train_step(self, x, target, scale, lr):
+    def train_step(self, x, d, target, scale, lr, update_scale=None):
+        us = scale * UPDATE_GAIN if update_scale is None else update_scale
         loss = C.c_double()
-        call("etn_unet_train_step", self._h, C.c_void_p(x.data_ptr()), C.c_void_p(target.data_ptr()), float(scale),
-             float(lr), C.byref(loss))
+        call("etn_unet_train_step", self._h, _p(x), _p(d), _p(target), float(scale), float(us), float(lr),
+             C.byref(loss))
         return loss.value
 
     def time_forward(self, x, scale):
         ms, flops = C.c_double(), C.c_double()
-        call("etn_unet_time_forward", self._h, C.c_void_p(x.data_ptr()), float(scale), C.byref(ms), C.byref(flops))
+        call("etn_unet_time_forward", self._h, _p(x), float(scale), C.byref(ms), C.byref(flops))
         return ms.value, flops.value
-
-
-def sart_pairs(members, k0, k1, flags=0):
-    """Runs the batch of reconstructions (keyword dicts as for
-    BlockIterativeReconstructionOperator) and returns float32 device volumes
-    (iterate k0, iterate k1) per member."""
-    b = A.BatchReconstruction(members, flags=flags)
-    b.run_iterations(k0)
-    early = [b.get_reconstructed_volume(i, dtype=_torch().float32) for i in range(len(members))]
-    b.run_iterations(k1 - k0)
-    late = [b.get_reconstructed_volume(i, dtype=_torch().float32) for i in range(len(members))]
-    b.close()
-    return early, late
-
-
-def train(net, pairs, scale, steps, lr=1e-3, log=None):
-    """Adam over the (early, late) pairs in a fixed cyclic order; returns the loss per step."""
-    losses = []
-    for s in range(steps):
-        x, t = pairs[s % len(pairs)]
-        losses.append(net.train_step(x, t, scale, lr))
-        if log and (s % 50 == 0 or s == steps - 1):
-            log("step %d loss %.6e" % (s, losses[-1]))
-    return losses
-
-
-class LearnedSart:
-    """SART with a learned jump from iterate k0: run k0 iterations, apply the
-    network, and keep the result only if ||Ax - b|| is below what plain SART
-    reaches in the same wall-clock time (the reconstruction's own FP64 norm)."""
-
-    def __init__(self, net, scale, k0=3):
-        self.net, self.scale, self.k0 = net, scale, k0
-
-    def run(self, make_reconstruction, plain_iteration_ms=None):
-        torch = _torch()
-        r = make_reconstruction()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r.run_iterations(self.k0)
-        x = r.get_reconstructed_volume(dtype=torch.float32)
-        y = self.net.predict(x, self.scale)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        r.set_volume(y.to(_voxel_dtype(r)))
-        corrected = r.residual_norm()
-        # plain SART given the same time: whole iterations that fit
-        p = make_reconstruction()
-        if plain_iteration_ms is None:
-            p.run_iterations(1)
-            p.synchronize()
-            t1 = time.perf_counter()
-            p.run_iterations(1)
-            p.synchronize()
-            plain_iteration_ms = 1e3 * (time.perf_counter() - t1)
-            p = make_reconstruction()
-        k = max(self.k0, int(elapsed * 1e3 / plain_iteration_ms))
-        p.run_iterations(k)
-        plain = p.residual_norm()
-        accepted = corrected < plain
-        return {"accepted": accepted, "corrected_norm": corrected, "plain_norm": plain, "plain_iterations": k,
-                "elapsed_ms": 1e3 * elapsed, "volume": y if accepted else p.get_reconstructed_volume(torch.float32)}
 
 
 def make_case(shape_zyx, ntilts, seed, counts=1e4, max_angle=60.0, device="cuda"):
@@ -205,28 +162,11 @@ def sart_iterates(case, iterations, voxel_type="float", lam=0.3, order="random")
     return out
 
 
-def train_on_crops(net, pairs, crop, steps, scale, lr=1e-3, seed=0, log=None):
-    """Adam steps on random crop^3 windows (offsets multiples of 4) of the
-    (early, late) pairs, cycling through the pairs; net's working size is crop^3."""
-    rng = np.random.default_rng(seed)
-    losses = []
-    for s in range(steps):
-        x, t = pairs[s % len(pairs)]
-        o = [4 * int(rng.integers(0, (n - crop) // 4 + 1)) for n in x.shape]
-        sl = tuple(slice(a, a + crop) for a in o)
-        losses.append(net.train_step(x[sl].contiguous(), t[sl].contiguous(), scale, lr))
-        if log and (s % 100 == 0 or s == steps - 1):
-            log("train step %d: loss %.4e" % (s, losses[-1]))
-    return losses
-
-
 def sart_best_iterate(case, k_max=50, voxel_type="float", lam=0.3, order="random", patience=4):
-    """SART as the reference implements it semi-converges: ||Ax - b|| falls for
-    the first iterations, then grows (the voxel-driven bilinear back projection
-    with its half-pixel offset is not the forward projector's adjoint).  Runs up
-    to k_max iterations, stopping `patience` iterations after the last new
-    minimum, and returns (float32 copy of the minimum-residual iterate, its
-    iteration number, the residual after every iteration run)."""
+    """Runs up to k_max SART iterations, stopping `patience` iterations after
+    the last new minimum of ||Ax - b||, and returns (float32 copy of the
+    minimum-residual iterate, its iteration number, the residual after every
+    iteration run)."""
     torch = _torch()
     r = A.BlockIterativeReconstructionOperator(algorithm="sart", lam=lam, projection_iteration=order,
                                                voxel_type=voxel_type, **case)
@@ -242,13 +182,50 @@ def sart_best_iterate(case, k_max=50, voxel_type="float", lam=0.3, order="random
     return best, best_k, norms
 
 
+def training_sample(case, k0=3, k_max=50, lam=0.3, order="random"):
+    """(x, d, target): bf16-storage SART iterate k0, its last update and the
+    FP32 SART minimum-residual iterate; plus the target's iteration number."""
+    prev, x = sart_iterates(case, [k0 - 1, k0], voxel_type="bfloat16", lam=lam, order=order)
+    target, k_best, _ = sart_best_iterate(case, k_max, lam=lam, order=order)
+    return (x, x - prev, target), k_best
+
+
+def train(net, samples, scale, steps, lr=1e-3, log=None):
+    """Adam over whole (x, d, target) samples in a fixed cyclic order; the loss per step."""
+    losses = []
+    for s in range(steps):
+        x, d, t = samples[s % len(samples)]
+        losses.append(net.train_step(x, d, t, scale, lr))
+        if log and (s % 100 == 0 or s == steps - 1):
+            log("train step %d: loss %.4e" % (s, losses[-1]))
+    return losses
+
+
+def train_on_crops(net, samples, crop, steps, scale, lr=1e-3, lr_min=5e-5, seed=0, log=None):
+    """Adam steps on random crop^3 windows (offsets multiples of 4) of the
+    (x, d, target) samples, cycling through them, with the learning rate on a
+    cosine from lr to lr_min; net's working size is crop^3."""
+    rng = np.random.default_rng(seed)
+    losses = []
+    for s in range(steps):
+        x, d, t = samples[s % len(samples)]
+        o = [4 * int(rng.integers(0, (n - crop) // 4 + 1)) for n in x.shape]
+        sl = tuple(slice(a, a + crop) for a in o)
+        rate = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * s / max(steps - 1, 1)))
+        losses.append(net.train_step(x[sl].contiguous(), d[sl].contiguous(), t[sl].contiguous(), scale, rate))
+        if log and (s % 100 == 0 or s == steps - 1):
+            log("train step %d: loss %.4e" % (s, losses[-1]))
+    return losses
+
+
 def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", patience=4, log=None):
     """Wall time until ||Ax - b|| (the reconstruction's FP64 norm, evaluated
     outside the timed regions) reaches the lowest value plain FP32 SART attains
     within k_max iterations: plain FP32 SART against bf16-storage SART that
     jumps with the network after k0 iterations.  Times are sums of synchronised
     iteration (and network) times; both runs stop `patience` iterations after
-    their last new minimum."""
+    their last new minimum.  The jump is `accepted` when its residual is below
+    plain SART's after the iterations that fit in the jump's wall-clock time."""
     torch = _torch()
 
     def timed(fn):
@@ -276,13 +253,14 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
                                                  voxel_type="bfloat16", **case)
     lrn.run_iterations(1)
     lrn.set_volume(torch.zeros(lrn.shape, dtype=torch.bfloat16, device="cuda"))
-    holder = {}
 
     def jump():
-        lrn.run_iterations(k0)
+        lrn.run_iterations(k0 - 1)
+        prev = lrn.get_reconstructed_volume(dtype=torch.float32)
+        lrn.run_iterations(1)
         x = lrn.get_reconstructed_volume(dtype=torch.float32)
-        holder["y"] = net.predict(x, scale)
-        lrn.set_volume(holder["y"].to(torch.bfloat16))
+        y = net.predict(x, x - prev, scale)
+        lrn.set_volume(y.to(torch.bfloat16))
 
     t_jump = timed(jump)
     jump_norm = lrn.residual_norm()
@@ -295,12 +273,12 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
             best_norm, since_best = norm, 0
         else:
             since_best += 1
-    # the acceptance rule: the jump against plain SART given the same wall-clock time
     t_iter = sum(p_times) / len(p_times)
     k_equal = min(max(int(t_jump / t_iter), 1), len(p_norms))
     accepted = jump_norm < p_norms[k_equal - 1]
     reached = norm <= target
     res = {"target_norm": target, "plain_best_iteration": k_best, "plain_s": t_plain,
+           "plain_norm_at_k0": p_norms[min(k0, len(p_norms)) - 1],
            "plain_iteration_ms": 1e3 * t_iter, "jump_iterations": k0, "jump_s": t_jump, "jump_norm": jump_norm,
            "plain_norm_at_jump_time": p_norms[k_equal - 1], "plain_iterations_at_jump_time": k_equal,
            "accepted": bool(accepted), "learned_extra_iterations": extra, "learned_s": t_learned,
@@ -312,6 +290,39 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
             "%d more iterations, %.3f s total, target %s" % (k0, t_jump, jump_norm, k_equal, p_norms[k_equal - 1],
                                                            extra, t_learned, "reached" if reached else "missed"))
     return res
+
+
+def learned_sart(case, net, scale, k0=3, lam=0.3, order="random", voxel_type="bfloat16"):
+    """SART with the learned jump from iterate k0, plain SART as the judge:
+    returns (volume, accepted, jump residual, plain residual), where plain SART
+    gets the iterations that fit in the jump's wall-clock time and its volume
+    is returned whenever the jump does not beat it."""
+    torch = _torch()
+    r = A.BlockIterativeReconstructionOperator(algorithm="sart", lam=lam, projection_iteration=order,
+                                               voxel_type=voxel_type, **case)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r.run_iterations(k0 - 1)
+    prev = r.get_reconstructed_volume(dtype=torch.float32)
+    r.run_iterations(1)
+    x = r.get_reconstructed_volume(dtype=torch.float32)
+    y = net.predict(x, x - prev, scale)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    r.set_volume(y.to(_voxel_dtype(r)))
+    jump_norm = r.residual_norm()
+    p = A.BlockIterativeReconstructionOperator(algorithm="sart", lam=lam, projection_iteration=order, **case)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    k = 0
+    while k == 0 or time.perf_counter() - t1 < elapsed:
+        p.run_iterations(1)
+        p.synchronize()
+        k += 1
+    plain_norm = p.residual_norm()
+    accepted = jump_norm < plain_norm
+    vol = y if accepted else p.get_reconstructed_volume(dtype=torch.float32)
+    return vol, accepted, jump_norm, plain_norm
 
 
 def _voxel_dtype(r):

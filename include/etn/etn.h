@@ -258,10 +258,11 @@ etn_status etn_batch_time_iteration(etn_batch* b, double* fused_ms, uint32_t* la
 /* ---------------------------------------------------------------------------
  * Learned correction (extension, no reference counterpart): a 3-level 3-D
  * U-Net (16/32/64 channels, 3x3x3 convolutions, bf16 on the matrix cores with
- * FP32 accumulation) mapping an early SART iterate x to a late one,
- * x + delta(x).  Volumes are device float32 [nz][ny][nx]; sizes are multiples
- * of 4.  `scale` multiplies the volume before the network and divides the
- * correction after it.  The flat parameter vector holds, per layer c1a c1b c2a
+ * FP32 accumulation) mapping an early SART iterate x and its last update d (x
+ * minus the iterate before it) to a late iterate, x + delta(x, d).  Volumes
+ * are device float32 [nz][ny][nx]; sizes are multiples of 4.  The network
+ * sees (x scale, d update_scale) and its correction is divided by scale; d
+ * may be NULL (read as zeros).  The flat parameter vector holds, per layer c1a c1b c2a
  * c2b c3a c3b c4a c4b c5a c5b, weights [cout][27][cin] then biases, then the
  * head's 16 weights and bias.
  * ------------------------------------------------------------------------- */
@@ -273,15 +274,16 @@ etn_status etn_unet_destroy(etn_unet* n);
 etn_status etn_unet_parameter_count(size_t* out);
 etn_status etn_unet_set_parameters(etn_unet* n, const float* host, size_t count);
 etn_status etn_unet_get_parameters(etn_unet* n, float* host, size_t count);
-/* out = x + delta(x scale) / scale; asynchronous on the network's stream. */
-etn_status etn_unet_predict(etn_unet* n, const float* x, float scale, float* out);
+/* out = x + delta(x scale, d update_scale) / scale; asynchronous on the network's stream. */
+etn_status etn_unet_predict(etn_unet* n, const float* x, const float* d, float scale, float update_scale,
+                            float* out);
 /* Forward + backward of L = mean((x scale + delta - target scale)^2); copies the
  * gradient (FP32, parameter order) to `grad_host` when not NULL. Synchronises. */
-etn_status etn_unet_gradient(etn_unet* n, const float* x, const float* target, float scale, float* grad_host,
-                             double* loss);
+etn_status etn_unet_gradient(etn_unet* n, const float* x, const float* d, const float* target, float scale,
+                             float update_scale, float* grad_host, double* loss);
 /* etn_unet_gradient followed by one Adam step with learning rate `lr`. */
-etn_status etn_unet_train_step(etn_unet* n, const float* x, const float* target, float scale, float lr,
-                               double* loss);
+etn_status etn_unet_train_step(etn_unet* n, const float* x, const float* d, const float* target, float scale,
+                               float update_scale, float lr, double* loss);
 /* HIP-event time of one forward pass and its convolution flops. */
 etn_status etn_unet_time_forward(etn_unet* n, const float* x, float scale, double* ms, double* flops);
 

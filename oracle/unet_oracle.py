@@ -1,5 +1,6 @@
 """TEST INFRASTRUCTURE ONLY -- FP64 numpy restatement of the learned
 correction network (3-D U-Net, 3 levels, 16/32/64 channels, 3x3x3 convs).
+Its input is the pair (x, d): an early SART iterate and its last update.
 
 The network has no counterpart in the reference: it is the north-star
 extension that predicts a late SART iterate from an early one.  This module is
@@ -19,7 +20,7 @@ import numpy as np
 # (name, input channels, output channels); inputs of c4a / c5a are
 # concat(upsample(lower level), skip) in that channel order.
 LAYERS = [
-    ("c1a", 1, 16), ("c1b", 16, 16),
+    ("c1a", 2, 16), ("c1b", 16, 16),
     ("c2a", 16, 32), ("c2b", 32, 32),
     ("c3a", 32, 64), ("c3b", 64, 64),
     ("c4a", 96, 32), ("c4b", 32, 32),
@@ -114,10 +115,13 @@ def relu(x):
     return np.maximum(x, 0.0)
 
 
-def forward(params, x):
-    """x: [D][H][W] (D, H, W multiples of 4).  Returns (prediction x + delta, cache)."""
+def forward(params, x, d=None):
+    """x, d: [D][H][W] (D, H, W multiples of 4; d None reads as zeros).
+    Returns (prediction x + delta, cache)."""
     P, (hw, hb) = unpack(params)
-    c = {"x": np.asarray(x, np.float64)[..., None]}
+    x = np.asarray(x, np.float64)
+    d = np.zeros_like(x) if d is None else np.asarray(d, np.float64)
+    c = {"x": np.stack([x, d], axis=-1)}
 
     def layer(name, inp):
         c[name + ".in"] = inp
@@ -134,10 +138,10 @@ def forward(params, x):
     return c["x"][..., 0] + delta, c
 
 
-def loss_and_gradient(params, x, target):
+def loss_and_gradient(params, x, target, d=None):
     """L = mean((prediction - target)^2) and dL/dparams (flat, FP64)."""
     P, (hw, hb) = unpack(params)
-    pred, c = forward(params, x)
+    pred, c = forward(params, x, d)
     r = pred - np.asarray(target, np.float64)
     L = float(np.mean(r * r))
     g = 2.0 * r / r.size

@@ -1,8 +1,8 @@
 """What the trained correction network gains over SART without it.
 
-The network is trained by the library's own kernels on (bf16 SART iterate 3,
-FP32 SART minimum-residual iterate) pairs of seeded 64^3 phantoms and applied
-to a held-out phantom.  Plain SART is the judge: the jump must lower the projection
+The network is trained by the library's own kernels on samples of seeded 64^3
+phantoms -- bf16 SART iterate 3 and its last update as input, the FP32 SART
+minimum-residual iterate as target -- and applied to a held-out phantom.  Plain SART is the judge: the jump must lower the projection
 residual ||Ax - b|| (FP64, the reconstruction's own norm) below that of the
 iterate it started from, and the acceptance rule must keep plain SART's
 result whenever the jump loses to the iterations plain SART runs in the same
@@ -28,14 +28,13 @@ def norm_of(case, volume):
 
 @pytest.fixture(scope="module")
 def trained():
-    pairs = []
+    samples = []
     for i in range(8):
         case, _ = L.make_case(SHAPE, TILTS, seed=6001 + i)
-        early = L.sart_iterates(case, [3], voxel_type="bfloat16")[0]
-        late, _, _ = L.sart_best_iterate(case, 50)
-        pairs.append((early, late))
+        sample, _ = L.training_sample(case)
+        samples.append(sample)
     net = L.CorrectionNet(SHAPE, seed=5000)
-    losses = L.train(net, pairs, 1.0, 300, lr=1e-3)
+    losses = L.train(net, samples, 1.0, 300, lr=1e-3)
     return net, losses
 
 
@@ -48,8 +47,8 @@ def test_training_lowers_the_loss(trained):
 def test_jump_lowers_the_projection_residual(trained):
     net, _ = trained
     case, _ = L.make_case(SHAPE, TILTS, seed=6000)
-    x3 = L.sart_iterates(case, [3], voxel_type="bfloat16")[0]
-    y = net.predict(x3, 1.0)
+    x2, x3 = L.sart_iterates(case, [2, 3], voxel_type="bfloat16")
+    y = net.predict(x3, x3 - x2, 1.0)
     before, after = norm_of(case, x3), norm_of(case, y)
     print("residual of iterate 3 %.6e, after the jump %.6e" % (before, after))
     assert after < before, (before, after)

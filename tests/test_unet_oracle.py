@@ -48,8 +48,8 @@ def test_pool_and_upsample_are_adjoint():
 
 
 def test_parameter_count():
-    # 448 + 6928 + 13856 + 27680 + 55360 + 110656 + 82976 + 27680 + 20752 + 6928 + 17
-    assert U.parameter_count() == len(U.init_parameters(0)) == 353_281
+    # 880 + 6928 + 13856 + 27680 + 55360 + 110656 + 82976 + 27680 + 20752 + 6928 + 17
+    assert U.parameter_count() == len(U.init_parameters(0)) == 353_713
 
 
 @pytest.mark.parametrize("shape", [(4, 4, 4), (8, 4, 4)])
@@ -65,8 +65,9 @@ def test_gradient_matches_central_differences(shape):
         params[at:at + co] = rng.normal(0.0, 0.05, co)
         at += co
     x = rng.uniform(0.0, 1.0, shape)
+    d = rng.normal(0.0, 0.5, shape)
     target = x + rng.normal(0.0, 0.1, shape)
-    L, g = U.loss_and_gradient(params, x, target)
+    L, g = U.loss_and_gradient(params, x, target, d)
     # two entries of every layer's weights, its bias, and the head
     picks = []
     at = 0
@@ -80,5 +81,5 @@ def test_gradient_matches_central_differences(shape):
         p1, p0 = params.copy(), params.copy()
         p1[i] += h
         p0[i] -= h
-        fd = (U.loss_and_gradient(p1, x, target)[0] - U.loss_and_gradient(p0, x, target)[0]) / (2 * h)
+        fd = (U.loss_and_gradient(p1, x, target, d)[0] - U.loss_and_gradient(p0, x, target, d)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
