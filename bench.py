@@ -434,25 +434,30 @@ def kernel_roofline(r, voxel_type, fused, V, P):
     achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
     log("dominant kernel %s: %.4f ms average over %d launches -> %.1f GB/s algorithmic"
         % (name, dur_ms, launches, achieved))
-    traffic, source = measured_traffic(kernel, voxel_type)
+    traffic, source, l2 = measured_traffic(kernel, voxel_type)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
+            "l2_hit_rate": l2, "kernel": name,
             "avg_launch_ms": round(dur_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes),
             "kernel_compulsory_bytes_per_launch": int(kernel_bytes), "timed_launches": launches}
 
 
 def measured_traffic(kernel, voxel_type):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), when one exists for it."""
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), and its L2 hit rate, when
+    the summary exists."""
     name = {0: "fused_plane_kernel", 1: "forward_project_kernel", 2: "back_project_kernel"}[kernel]
     vt = {"float": "f32", "half": "f16", "bfloat16": "bf16"}[voxel_type]
     path = os.path.join(ROOT, "profiles", "%s_sart_512cubed_121_%s_traffic.md" % (name, vt))
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
+    traffic = l2 = None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+            traffic = int(line.split(":")[1])
+        if line.startswith("l2_hit_rate:"):
+            l2 = float(line.split(":")[1])
+    return traffic, os.path.relpath(path, ROOT), l2
 
 
 def cpu_baseline(wl, tilts, projs, nsample):

@@ -19,6 +19,7 @@ constexpr LayerSpec kLayers[] = {
 };
 constexpr int kLayerCount = 10;
 constexpr int kHead = 16;
+constexpr int kInputChannels = 8;   // (x, d) padded to one 16-byte channel group
 
 }  // namespace
 
@@ -49,11 +50,11 @@ UNet3D::UNet3D(int nx, int ny, int nz, hipStream_t stream) : stream_(stream)
     params_.allocate(sizeof(float) * P);
     ETN_HIP(hipMemsetAsync(params_.get(), 0, sizeof(float) * P, stream_));
     for (const Layer& L : layers_) {
-        wt_.emplace_back(sizeof(uint16_t) * L.cout * nn::kpad_for(L.cin));
+        wt_.emplace_back(sizeof(uint16_t) * L.cout * nn::kpad_for(std::max(L.cin, kInputChannels)));
         wb_.emplace_back(L.cin >= 8 ? sizeof(uint16_t) * L.cin * nn::kpad_for(L.cout) : 0);
         act_.emplace_back(sizeof(uint16_t) * voxels(L.level) * L.cout);
     }
-    xin_.allocate(sizeof(uint16_t) * voxels(0) * 2);
+    xin_.allocate(sizeof(uint16_t) * voxels(0) * kInputChannels);
     pool1_.allocate(sizeof(uint16_t) * voxels(1) * 16);
     pool2_.allocate(sizeof(uint16_t) * voxels(2) * 32);
     pred_.allocate(sizeof(float) * voxels(0));
@@ -84,8 +85,8 @@ void UNet3D::packAll()
 {
     for (int i = 0; i < kLayerCount; ++i) {
         const Layer& L = layers_[i];
-        nn::pack_weights(params_.as<float>() + L.w, L.cout, L.cin, wt_[i].as<uint16_t>(),
-                         L.cin >= 8 ? wb_[i].as<uint16_t>() : nullptr, stream_);
+        nn::pack_weights(params_.as<float>() + L.w, L.cout, L.cin, std::max(L.cin, kInputChannels),
+                         wt_[i].as<uint16_t>(), L.cin >= 8 ? wb_[i].as<uint16_t>() : nullptr, stream_);
     }
 }
 
@@ -93,7 +94,7 @@ nn::ConvSrc UNet3D::layerInput(int i) const
 {
     nn::ConvSrc s;
     switch (i) {
-        case 0: s.a = xin_.as<uint16_t>(); s.ca = 2; break;
+        case 0: s.a = xin_.as<uint16_t>(); s.ca = kInputChannels; break;
         case 2: s.a = pool1_.as<uint16_t>(); s.ca = 16; break;
         case 4: s.a = pool2_.as<uint16_t>(); s.ca = 32; break;
         case 6:
@@ -122,7 +123,7 @@ void UNet3D::forward(const float* x, const float* d, float scale, float dscale)
         a.H = H_[L.level];
         a.W = W_[L.level];
         a.wt = wt_[i].as<uint16_t>();
-        a.kpad = nn::kpad_for(L.cin);
+        a.kpad = nn::kpad_for(std::max(L.cin, kInputChannels));
         a.bias = params_.as<float>() + L.b;
         a.cout = L.cout;
         a.relu = 1;

@@ -40,9 +40,7 @@ constexpr int kConvWaves = 4;
 constexpr int kThreads = 256;
 
 // ---- 3x3x3 convolution, implicit GEMM ---------------------------------------
-// kSmall: the network input, ca < 8 channels and no b; k = tap * ca + ci
-// gathers single values (the 27 ca products are padded to a multiple of 32)
-template <int NT, int MT, bool kSmall>
+template <int NT, int MT>
 __global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -80,24 +78,7 @@ __global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) B[nt] = as_bf16x8(load16(a.wt + (size_t)(nt * 16 + row) * a.kpad + k0));
         bf16x8 A[MT];
-        if (kSmall) {
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                uint16_t v[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int t = (k0 + e) / ca, ci = k0 + e - t * ca;
-                    v[e] = 0;
-                    if (t < 27) {
-                        const int zz = vz[mt] + t / 9 - 1, yy = vy[mt] + (t / 3) % 3 - 1, xx = vx[mt] + t % 3 - 1;
-                        if ((unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W)
-                            v[e] = a.src.a[(((size_t)zz * a.H + yy) * a.W + xx) * ca + ci];
-                    }
-                }
-                A[mt] = as_bf16x8(u32x4{v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16),
-                                        v[4] | ((uint32_t)v[5] << 16), v[6] | ((uint32_t)v[7] << 16)});
-            }
-        } else {
+        {
             const int t = k0 / cin, ci = k0 - t * cin;
             const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
 #pragma unroll
@@ -148,10 +129,7 @@ void launch_conv(const ConvArgs& a, hipStream_t stream)
     const size_t M = (size_t)a.D * a.H * a.W;
     const size_t per_block = (size_t)kConvWaves * MT * 16;
     const dim3 grid((unsigned)((M + per_block - 1) / per_block));
-    if (a.src.ca < 8 && a.src.cb == 0)
-        conv3_fwd_kernel<NT, MT, true><<<grid, kThreads, 0, stream>>>(a);
-    else
-        conv3_fwd_kernel<NT, MT, false><<<grid, kThreads, 0, stream>>>(a);
+    conv3_fwd_kernel<NT, MT><<<grid, kThreads, 0, stream>>>(a);
 }
 
 // ---- weight gradient ----------------------------------------------------------
@@ -168,7 +146,6 @@ struct WgradArgs {
     float* partial;     // [chunks][27][cout][cin16]
 };
 
-template <bool kSmall>
 __global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
 {
     __shared__ __attribute__((aligned(16))) uint16_t gyT[kMaxCout * kLdsRow];
@@ -184,9 +161,6 @@ __global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
     const int nco = w.cout / 16, nci = cin16 / 16, ntiles = nco * nci;
     const int Hh = w.H >> 1, Wh = w.W >> 1;
     const size_t mbeg = (size_t)blockIdx.x * w.chunk;
-    if (kSmall) {   // rows ca..15 of xT stay zero
-        for (int i = threadIdx.x; i < 16 * kLdsRow; i += kThreads) xT[i] = 0;
-    }
     floatx4 acc[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -207,20 +181,7 @@ __global__ void __launch_bounds__(kThreads) conv3_wgrad_kernel(WgradArgs w)
             }
         }
         // tap-shifted input, transposed: xT[ci][voxel]
-        if (kSmall) {
-            for (int g = threadIdx.x; g < 32 * ca; g += kThreads) {
-                const int mi = g / ca, ci = g - mi * ca;
-                const size_t m = ms + mi;
-                uint16_t v = 0;
-                if (m < M) {
-                    const int z = (int)(m / HW), r = (int)(m - (size_t)z * HW), y = r / w.W, x = r - y * w.W;
-                    const int zz = z + dz, yy = y + dy, xx = x + dx;
-                    if ((unsigned)zz < (unsigned)w.D && (unsigned)yy < (unsigned)w.H && (unsigned)xx < (unsigned)w.W)
-                        v = w.src.a[(((size_t)zz * w.H + yy) * w.W + xx) * ca + ci];
-                }
-                xT[ci * kLdsRow + mi] = v;
-            }
-        } else {
+        {
             const int gpx = cin / 8;
             for (int g = threadIdx.x; g < 32 * gpx; g += kThreads) {
                 const int mi = g / gpx, c0 = (g - mi * gpx) * 8;
@@ -313,13 +274,13 @@ __global__ void sum_partials_kernel(const float* partial, int chunks, int C, flo
     out[c] = s;
 }
 
-__global__ void pack_weights_kernel(const float* w, int cout, int cin, uint16_t* wt, int kpad, uint16_t* wb,
-                                    int kpad_b)
+__global__ void pack_weights_kernel(const float* w, int cout, int cin, int cin_pad, uint16_t* wt, int kpad,
+                                    uint16_t* wb, int kpad_b)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cout * kpad) {
-        const int co = i / kpad, k = i - co * kpad, t = k / cin, ci = k - t * cin;
-        wt[i] = t < 27 ? f2bf(w[((size_t)co * 27 + t) * cin + ci]) : (uint16_t)0;
+        const int co = i / kpad, k = i - co * kpad, t = k / cin_pad, ci = k - t * cin_pad;
+        wt[i] = (t < 27 && ci < cin) ? f2bf(w[((size_t)co * 27 + t) * cin + ci]) : (uint16_t)0;
     }
     if (wb && i < cin * kpad_b) {
         const int ci = i / kpad_b, k = i - ci * kpad_b, t = k / cout, co = k - t * cout;
@@ -397,8 +358,8 @@ __global__ void take_channels_kernel(const uint16_t* g, int stride, int c0, size
 __global__ void pack_input_kernel(const float* x, float scale, const float* d, float dscale, size_t n, uint16_t* y)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        y[2 * i] = f2bf(x[i] * scale);
-        y[2 * i + 1] = d ? f2bf(d[i] * dscale) : (uint16_t)0;
+        const uint32_t a = f2bf(x[i] * scale), b = d ? f2bf(d[i] * dscale) : 0u;
+        *reinterpret_cast<u32x4*>(y + 8 * i) = u32x4{a | (b << 16), 0u, 0u, 0u};
     }
 }
 
@@ -502,9 +463,7 @@ unsigned grid_for(size_t n)
 
 void conv3_forward(const ConvArgs& a, hipStream_t stream)
 {
-    const bool small = a.src.ca < 8 && a.src.cb == 0;
-    if (a.src.ca < 1 || (!small && ((a.src.ca % 8) || (a.src.cb % 8))))
-        fail("conv3: input channels must be multiples of 8 (or fewer than 8 without a second source)");
+    if (a.src.ca < 8 || (a.src.ca % 8) || (a.src.cb % 8)) fail("conv3: input channels must be multiples of 8");
     if (a.kpad != kpad_for(a.src.ca + a.src.cb)) fail("conv3: weight row length mismatch");
     if (a.src.a_half && ((a.D | a.H | a.W) & 1)) fail("conv3: upsampled input needs even sizes");
     switch (a.cout) {
@@ -521,13 +480,12 @@ void conv3_forward(const ConvArgs& a, hipStream_t stream)
 void conv3_weight_grad(const ConvSrc& src, int D, int H, int W, const uint16_t* gy, int cout, float* partial,
                        hipStream_t stream)
 {
-    const bool small = src.ca < 8 && src.cb == 0;
-    if (cout % 16 || cout > kMaxCout || src.ca + src.cb > kMaxCin) fail("conv3 weight gradient: channel counts");
+    if (cout % 16 || cout > kMaxCout || src.ca + src.cb > kMaxCin || src.ca % 8 || src.cb % 8)
+        fail("conv3 weight gradient: channel counts");
     WgradArgs w{src, D, H, W, gy, cout, kWgradChunk, partial};
     const size_t M = (size_t)D * H * W;
     const dim3 grid((unsigned)wgrad_chunks(M), 27);
-    if (small) conv3_wgrad_kernel<true><<<grid, kThreads, 0, stream>>>(w);
-    else conv3_wgrad_kernel<false><<<grid, kThreads, 0, stream>>>(w);
+    conv3_wgrad_kernel<<<grid, kThreads, 0, stream>>>(w);
     hip_check(hipGetLastError(), "conv3 weight gradient launch");
 }
 
@@ -548,11 +506,12 @@ void channel_sum(const uint16_t* g, size_t voxels, int channels, float* partial,
     hip_check(hipGetLastError(), "channel sum launch");
 }
 
-void pack_weights(const float* w, int cout, int cin, uint16_t* wt, uint16_t* wb, hipStream_t stream)
+void pack_weights(const float* w, int cout, int cin, int cin_pad, uint16_t* wt, uint16_t* wb, hipStream_t stream)
 {
-    const int kpad = kpad_for(cin), kpad_b = kpad_for(cout);
+    const int kpad = kpad_for(cin_pad), kpad_b = kpad_for(cout);
     const int n = std::max(cout * kpad, wb ? cin * kpad_b : 0);
-    pack_weights_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, stream>>>(w, cout, cin, wt, kpad, wb, kpad_b);
+    pack_weights_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, stream>>>(w, cout, cin, cin_pad, wt, kpad, wb,
+                                                                               kpad_b);
     hip_check(hipGetLastError(), "weight packing launch");
 }
 

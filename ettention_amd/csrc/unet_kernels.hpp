@@ -14,8 +14,8 @@ namespace nn {
 
 // Input of a convolution: channels [0, ca) from `a` (at half resolution,
 // read through nearest-neighbour upsampling, when a_half), then channels
-// [ca, ca + cb) from `b` at full resolution.  ca and cb are multiples of 8,
-// or ca < 8 with no b (the network input).
+// [ca, ca + cb) from `b` at full resolution.  ca and cb are multiples of 8
+// (the network input is padded to 8 channels).
 struct ConvSrc {
     const uint16_t* a = nullptr;
     int ca = 0;
@@ -52,9 +52,10 @@ void conv3_weight_grad_reduce(const float* partial, int chunks, int cout, int ci
 void channel_sum(const uint16_t* g, size_t voxels, int channels, float* partial, float* out, hipStream_t stream);
 
 // packing of fp32 master weights [cout][27][cin] into the forward layout
-// (wt[co][k]) and the flipped, transposed backward-data layout
-// (wb[ci][tap' * cout + co] = w[co][26 - tap'][ci]), both bf16 and zero padded
-void pack_weights(const float* w, int cout, int cin, uint16_t* wt, uint16_t* wb, hipStream_t stream);
+// (wt[co][tap * cin_pad + ci], zero for ci >= cin) and the flipped, transposed
+// backward-data layout (wb[ci][tap' * cout + co] = w[co][26 - tap'][ci]), both
+// bf16 and zero padded to whole 32-wide k steps
+void pack_weights(const float* w, int cout, int cin, int cin_pad, uint16_t* wt, uint16_t* wb, hipStream_t stream);
 
 void relu_backward(uint16_t* g, const uint16_t* y, size_t count, hipStream_t stream);
 void pool2(const uint16_t* x, int D, int H, int W, int C, uint16_t* y, hipStream_t stream);
@@ -68,7 +69,7 @@ void up2_backward(const uint16_t* g, int stride, int c0, int D, int H, int W, in
 void take_channels(const uint16_t* g, int stride, int c0, size_t voxels, int C, uint16_t* dst, int accumulate,
                    hipStream_t stream);
 
-// y[m] = (x[m] scale, d[m] dscale) as two bf16 channels; d may be null (zeros)
+// y[m] = (x[m] scale, d[m] dscale, 0, 0, 0, 0, 0, 0) as eight bf16 channels; d may be null (zeros)
 void pack_input(const float* x, float scale, const float* d, float dscale, size_t count, uint16_t* y,
                 hipStream_t stream);
 // pred[m] = x[m] * scale + (sum_c hw[c] d1[m][c] + hb), fp32

@@ -329,3 +329,22 @@ def _voxel_dtype(r):
     torch = _torch()
     from ._lib import ETN_VOXEL_BFLOAT16, ETN_VOXEL_FLOAT16
     return {ETN_VOXEL_FLOAT16: torch.float16, ETN_VOXEL_BFLOAT16: torch.bfloat16}.get(r.voxel_type, torch.float32)
+
+
+def _main():
+    """python -m ettention_amd.learned [edge [repeats]]: HIP-event time of the
+    network's forward pass on an edge^3 volume of uniform noise."""
+    import sys
+    torch = _torch()
+    edge = int(sys.argv[1]) if len(sys.argv) > 1 else 512
+    repeats = int(sys.argv[2]) if len(sys.argv) > 2 else 3
+    net = CorrectionNet((edge, edge, edge))
+    x = torch.rand((edge, edge, edge), device="cuda")
+    for i in range(repeats):
+        ms, flops = net.time_forward(x, 1.0)
+        print("forward %d^3 run %d: %.3f ms, %.4e flop, %.1f TFLOP/s" % (edge, i, ms, flops, flops / ms / 1e9),
+              flush=True)
+
+
+if __name__ == "__main__":
+    _main()
