@@ -138,7 +138,8 @@ def main():
     norm = r.residual_norm()
     log("residual norm after %d iterations: %.6e" % (args.warmup + args.steps, norm))
 
-    roofline = kernel_roofline(r, args.voxel_type, fused, V, P)
+    block = wl["block_size"] or (1 if wl["algorithm"] == "sart" else N)
+    roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block)
     del r
     torch.cuda.empty_cache()
 
@@ -409,22 +410,24 @@ def run_learned(args):
     print(json.dumps(out), flush=True)
 
 
-def kernel_roofline(r, voxel_type, fused, V, P):
+def kernel_roofline(r, voxel_type, fused, V, P, block=1):
     """Average duration of the dominant kernel, measured with HIP events around
     every launch of one extra eager iteration on the reconstruction's stream,
     and its algorithmic bytes per launch.
 
-    Fused plane kernel: one launch back-projects projection k-1 and forward
-    projects projection k, i.e. V voxel updates; its algorithmic bytes are the
-    per-update figure of the unfused SART schedule (3 s + 20 P / V bytes per
-    update: the FP reads the volume, the BP reads and writes it, 20 bytes per
-    detector pixel) times V.  Generic FP: s V + 8 P; generic BP: 2 s V + 8 P."""
+    Fused plane kernel: one launch back-projects block b-1 and forward projects
+    block b (B projections each, B = 1 for SART), i.e. B V voxel updates; its
+    algorithmic bytes are the per-block figure of the unfused schedule,
+    3 s V + 20 P B (the FP reads the volume, the BP reads and writes it, 20
+    bytes per detector pixel and projection).  Generic FP: s V + 8 P; generic
+    BP: 2 s V + 8 P."""
     s = 4 if voxel_type == "float" else 2
     kernel, dur_ms, launches = r.time_kernels()
     if kernel == 0:
-        name = "fused_plane_kernel (BP of projection k-1 + FP/residual of projection k)"
-        alg_bytes = 3 * s * V + 20 * P
-        kernel_bytes = 2 * s * V + 28 * P
+        name = ("fused_plane_kernel (BP of projection k-1 + FP/residual of projection k)" if block == 1 else
+                "fused_plane_kernel (BP of block b-1 + FP/residual of block b, %d projections each)" % block)
+        alg_bytes = 3 * s * V + 20 * P * block
+        kernel_bytes = 2 * s * V + 28 * P * block
     elif kernel == 1:
         name, alg_bytes = "forward_project_kernel (generic DDA)", s * V + 8 * P
         kernel_bytes = alg_bytes
