@@ -242,8 +242,10 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // (1 ulp; OpenCL allows 2.5 ulp for a division), and the ray sums use fused
 // multiply-adds.  Without kFast the kernel performs the generic kernels'
 // float operations one for one.
+// With at most two rays per thread the kernel fits 128 VGPRs: four waves per
+// SIMD, two 512-thread workgroups per CU (as LDS also allows).
 template <typename S, bool kVec, bool kCap, bool kFast, int RPT>
-__global__ void __launch_bounds__(kFusedThreads)
+__global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
 {
