@@ -76,6 +76,8 @@ struct FusedVolume {
     int planes;                  // consecutive y planes per workgroup (divides ny)
     int band;                    // z rows per LDS band
     int rows_per_plane;          // detector rows whose rays lie in a plane, maximum over planes
+    int row_len;                 // staged columns per detector row, padding included
+    int pad_l;                   // zero columns staged left of detector column 0
     const int* row_lo;           // per plane: first detector row whose rays lie in it
     const int* row_hi;           // per plane: one past the last such row
     const int* dead_rows;        // detector rows whose rays all miss the volume in y
@@ -88,11 +90,12 @@ struct FusedWork {
     int plane;                   // first plane of the workgroup's group
 };
 
-// LDS floats a workgroup uses: the band, then (16-byte aligned) one 32-byte
-// record per padded detector column and plane, and a (m4 py, ty) pair per plane.
-__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int nu)
+// LDS floats a workgroup uses: the band, then (16-byte aligned) the staged
+// detector records of each plane (row_floats per plane: 8 floats per padded
+// column with exact arithmetic, 4 with fast) and a (m4 py, ty) pair per plane.
+__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats)
 {
-    return ((band * planes * nx + 3) & ~3) + planes * (8 * (nu + 2 * kRowPad) + 2);
+    return ((band * planes * nx + 3) & ~3) + planes * (row_floats + 2);
 }
 
 namespace {
@@ -131,6 +134,31 @@ __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, in
     int li = (r.vz - zlo) * width + r.off + r.vx;
     int vx = r.vx, vz = r.vz;
     float rx = r.rx, rz = r.rz, acc = r.acc, accl = r.accl;
+    if (!kCap) {
+        // one exit condition; whether the ray left the volume or only the band
+        // is decided after the loop (the same steps as the loop below)
+        while ((unsigned)vx < (unsigned)nx && (unsigned)(vz - zlo) < h) {
+            const bool ix = rx <= rz;
+            const float dt = ix ? rx : rz;
+            if (kFast) acc = fmaf(dt, band[li], acc);
+            else acc += dt * band[li];
+            accl += dt;
+            const float rest = (ix ? rz : rx) - dt;
+            rx = ix ? r.stepx : rest;
+            rz = ix ? rest : r.stepz;
+            vx += ix ? r.sx : 0;
+            vz += ix ? 0 : r.sz;
+            li += ix ? r.sx : lz;
+        }
+        r.live = ((unsigned)vx < (unsigned)nx && (unsigned)vz < (unsigned)nz) ? 1 : 0;
+        r.vx = vx;
+        r.vz = vz;
+        r.rx = rx;
+        r.rz = rz;
+        r.acc = acc;
+        r.accl = accl;
+        return;
+    }
     for (;;) {
         const bool inx = (unsigned)vx < (unsigned)nx;
         if (!(inx && (unsigned)(vz - zlo) < h)) {
@@ -260,13 +288,13 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     const int tid = threadIdx.x;
     const int nx = fv.nx, nz = fv.nz, nu = fv.nu, np = fv.planes;
     const int width = np * nx;                          // one band row: the x rows of the np planes
-    const int row_len = nu + 2 * kRowPad;
+    const int row_len = fv.row_len, pad_l = fv.pad_l;
     float* band = lds;                                  // band x width
     // per plane row_len column records, 16-byte aligned after the band, then
     // per plane (m4 py, ty) of the current back projection
     float4* recs = reinterpret_cast<float4*>(lds + ((fv.band * width + 3) & ~3));
-    float2* pterm = reinterpret_cast<float2*>(recs + np * 2 * row_len);
     constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
+    float2* pterm = reinterpret_cast<float2*>(recs + np * kRecStride * row_len);
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
     const size_t plane_stride = (size_t)nx * fv.ny;
     const size_t plane_base = (size_t)j * nx;
@@ -353,7 +381,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
         const ETN_GLOBAL float* res = global_ptr(bp.residual);
         const ETN_GLOBAL float* len = global_ptr(bp.length);
         for (int e = tid; e < np * row_len; e += kFusedThreads) {
-            const int p = e / row_len, k = e - p * row_len, u = k - kRowPad;
+            const int p = e / row_len, k = e - p * row_len, u = k - pad_l;
             float py, ty;
             const int row = plane_v(bp, p, py, ty);
             float v[8];   // (value, difference) of residual row, row + 1, length row, row + 1
@@ -453,32 +481,46 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 stage_rows(q);
                 __syncthreads();
             }
+            // the pass lives in global memory: copy what the loops use into registers
             const ETN_GLOBAL FusedBp& bp = pass->bp[q];
-            const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda, chord = bp.sample.w;
-            const float lc = lam * chord;
+            const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda;
+            const float4 smp = bp.sample;
+            const float chord = smp.w, lc = lam * chord;
             // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
             auto x_term = [&](int x, float a4) {
-                const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + bp.sample.x;
+                const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + smp.x;
                 return m0 * px + a4;
             };
             auto z_term = [&](int zz) {
-                const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + bp.sample.z;
+                const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + smp.z;
                 return m8 * pz;
             };
             // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
             auto update = [&](const float4* pr, float ty, float xt, float zt, float value) {
-                float u = xt + zt;
-                u = u + m12 * 1.0f;
-                float bu;
-                const float tx = cl_fract(u - 0.5f, &bu);
-                const int i = padded_index((int)bu, nu);
                 if (kFast) {
+                    // u in the reference's order (u is a few hundred pixels, so a
+                    // reordered sum would move the weights by ~1e-4); fract without
+                    // the clamp below 1 (at tx == 1 the interpolation gives the next
+                    // column's value, its limit).  The staged rows are padded (on the
+                    // host) beyond every column the member's projections reach, so the
+                    // index needs no clamp.
+                    float u = xt + zt;
+                    u = u + m12 * 1.0f;
+                    u = u - 0.5f;
+                    const float bu = floorf(u);
+                    const float tx = u - bu;
+                    const int i = (int)bu + pad_l;
                     const float4 c = pr[i];
                     const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
                     const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
                     value = (L > kEpsilon) ? value + contrib : value;
                     return S::round(value);
                 }
+                float u = xt + zt;
+                u = u + m12 * 1.0f;
+                float bu;
+                const float tx = cl_fract(u - 0.5f, &bu);
+                const int i = padded_index((int)bu, nu);
                 const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
                 const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
                 const float L = L0 + (L1 - L0) * ty;
@@ -665,6 +707,8 @@ struct MemberPlan {
     float lambda = 0.0f;
     bool use_loc = false;
     int rows_per_plane = 0;
+    int row_len = 0, pad_l = 0;        // staged detector columns, zero columns left of column 0
+    int row_floats = 0;                // LDS floats of one plane's staged rows
     int ndead = 0;
     uint32_t bmax = 0;                 // largest block
     DeviceBuffer rows;                 // row_lo[ny] | row_hi[ny] | dead rows
@@ -686,7 +730,7 @@ int max_band(const MemberPlan& m, int planes)
 {
     if (planes < 1 || planes > dev::kMaxPlanes || (planes & (planes - 1)) || m.ny % planes) return 0;
     if (planes * m.rows_per_plane * m.nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return 0;
-    int band = std::min(m.nz, (dev::kLdsFloats - 3 - planes * (8 * (m.nu + 2 * dev::kRowPad) + 2)) / (planes * m.nx));
+    int band = std::min(m.nz, (dev::kLdsFloats - 3 - planes * (m.row_floats + 2)) / (planes * m.nx));
     band = std::min(band, dev::kMaxBandFloats / (planes * m.nx));
     return std::max(band, 0);
 }
@@ -694,7 +738,7 @@ int max_band(const MemberPlan& m, int planes)
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
-           dev::fused_lds_floats(t.planes, t.band, m.nx, m.nu) <= dev::kLdsFloats;
+           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats) <= dev::kLdsFloats;
 }
 
 // Without timing: the fewest planes whose rays fill a workgroup, the tallest band.
@@ -724,7 +768,8 @@ std::vector<Tile> candidate_tiles(const MemberPlan& m)
 
 // Eligibility of one reconstruction and its plane/row tables; false when it
 // has no fused form.
-bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d, MemberPlan& out)
+bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d, bool exact,
+                 MemberPlan& out)
 {
     const VolumeProperties& vp = op.getReconstructedVolume()->properties();
     if (vp.subvolume_count() != 1) return false;
@@ -732,7 +777,6 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     if (d.mask) return false;
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
     const int nu = (int)d.base_geometry.resolution[0], nv = (int)d.base_geometry.resolution[1];
-    if (dev::fused_lds_floats(1, 1, nx, nu) > dev::kLdsFloats) return false;   // a z row and the detector rows fit
 
     // every projection: rays in y planes, constant detector row per plane
     std::vector<int> plane_of_row;
@@ -806,6 +850,36 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     out.lambda = d.lambda;
     out.use_loc = d.use_long_object_compensation != 0;
     out.rows_per_plane = rows_per_plane;
+    out.pad_l = dev::kRowPad;
+    out.row_len = nu + 2 * dev::kRowPad;
+    if (!exact) {
+        // the fast lookup skips the column clamp: stage every column u - 0.5 can
+        // floor to for a voxel centre of this member, plus one column either side
+        const VolumeProperties& vp = op.getReconstructedVolume()->properties();
+        const Vec3f vs = vp.voxel_size(), lo = vp.bbox_min(), hi = vp.bbox_max();
+        double umin = 1e300, umax = -1e300;
+        for (uint32_t b = 0; b < op.GetBlockCount(); ++b)
+            for (uint32_t p = 0; p < op.getBlockSize(b); ++p) {
+                const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
+                const Matrix4x4 pm = g.projection_matrix();
+                const float* mv = pm.data();
+                const std::vector<VoxelSample> smp = bp_sample_pattern(g, vs, 1);
+                for (int c = 0; c < 8; ++c) {
+                    const double px = ((c & 1) ? hi.x - 0.5 * vs.x : lo.x + 0.5 * vs.x) + smp[0].dx;
+                    const double py = ((c & 2) ? hi.y - 0.5 * vs.y : lo.y + 0.5 * vs.y) + smp[0].dy;
+                    const double pz = ((c & 4) ? hi.z - 0.5 * vs.z : lo.z + 0.5 * vs.z) + smp[0].dz;
+                    const double u = mv[0] * px + mv[4] * py + mv[8] * pz + mv[12];
+                    umin = std::min(umin, u);
+                    umax = std::max(umax, u);
+                }
+            }
+        if (!(umin > -1e6 && umax < 1e6)) return false;
+        const int bu_lo = (int)std::floor(umin - 0.5) - 1, bu_hi = (int)std::floor(umax - 0.5) + 1;
+        out.pad_l = std::max(dev::kRowPad, -bu_lo);
+        out.row_len = out.pad_l + std::max(nu + dev::kRowPad, bu_hi + 1);
+    }
+    out.row_floats = out.row_len * (exact ? 8 : 4);
+    if (dev::fused_lds_floats(1, 1, nx, out.row_floats) > dev::kLdsFloats) return false;   // a z row and the rows fit
     if (max_band(out, 1) < 1) return false;   // not even one z row of one plane fits
     out.ndead = (int)dead.size();
     std::vector<int> table(lo);
@@ -970,6 +1044,8 @@ void FusedEngineImpl::upload(hipStream_t stream)
         fv.planes = m.tile.planes;
         fv.band = m.tile.band;
         fv.rows_per_plane = m.rows_per_plane;
+        fv.row_len = m.row_len;
+        fv.pad_l = m.pad_l;
         fv.row_lo = m.rows.as<int>();
         fv.row_hi = m.rows.as<int>() + m.ny;
         fv.dead_rows = m.rows.as<int>() + 2 * m.ny;
@@ -1051,7 +1127,8 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
             tables[k * members_.size() + i] = base + pass_base_[i] + pi;
             for (int j = 0; j < m.ny; j += m.tile.planes) work.push_back({(int)i, j});
             l.rpt = std::max(l.rpt, select_rpt(p.nfp * m.tile.planes * m.rows_per_plane * m.nu));
-            l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx, m.nu));
+            l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx,
+                                                                                   m.row_floats));
             l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
         }
         l.groups = (int)(work.size() - l.work);
@@ -1214,7 +1291,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
         if (!plans.empty() && (d.voxel_type != vt || (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
             continue;
         MemberPlan m;
-        if (!plan_member(*members[i].op, d, m)) continue;
+        if (!plan_member(*members[i].op, d, exact, m)) continue;
         if (plans.empty()) {
             vt = d.voxel_type;
             vec = m.nx % 4 == 0;
