@@ -149,6 +149,18 @@ SIGNATURES = [
     ("etn_unet_gradient", _i32, [_vp, _vp, _vp, _vp, _f32, _f32, _vp, C.POINTER(C.c_double)]),
     ("etn_unet_train_step", _i32, [_vp, _vp, _vp, _vp, _f32, _f32, _f32, C.POINTER(C.c_double)]),
     ("etn_unet_time_forward", _i32, [_vp, _vp, _f32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("etn_projection_stack_open", _i32, [C.c_char_p, C.c_char_p, _i32, C.POINTER(_vp)]),
+    ("etn_projection_stack_destroy", _i32, [_vp]),
+    ("etn_projection_stack_info", _i32, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32), _SG]),
+    ("etn_projection_stack_read", _i32, [_vp, _vp, _vp]),
+    ("etn_image_stack_directory_write", _i32, [C.c_char_p, _vp, _u32, _u32, _u32, _vp, _SG, _vp]),
+    ("etn_tiff_read", _i32, [C.c_char_p, _u32, C.POINTER(_u32), C.POINTER(_u32), _vp]),
+    ("etn_tiff_page_count", _i32, [C.c_char_p, C.POINTER(_u32)]),
+    ("etn_tiff_write", _i32, [C.c_char_p, _vp, _u32, _u32]),
+    ("etn_mrc_write_volume", _i32, [C.c_char_p, _vp, _vp, C.c_char_p, C.c_char_p, _i32]),
+    ("etn_mrc_read_volume", _i32, [C.c_char_p, C.c_char_p, _vp, _vp]),
+    ("etn_config_resolve", _i32, [_i32, C.POINTER(C.c_char_p), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("etn_main", _i32, [_i32, C.POINTER(C.c_char_p)]),
 ]
 
 _lib = None

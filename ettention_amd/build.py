@@ -1,4 +1,5 @@
-"""Builds libetn.so (HIP kernels + C++ host layer + C ABI) in-tree for gfx950.
+"""Builds libetn.so (HIP kernels + C++ host layer + C ABI) in-tree for gfx950,
+and the command-line tool ettention_amd/etn linked against it.
 
     python -m ettention_amd.build
 
@@ -16,6 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libetn.so")
+CLI = os.path.join(PKG, "etn")
 OBJ = os.path.join(PKG, "_build")
 
 SOURCES = [
@@ -28,7 +30,12 @@ SOURCES = [
     "block_iterative.cpp",
     "batch.cpp",
     "unet.cpp",
+    "xml.cpp",
+    "io.cpp",
+    "config.cpp",
+    "framework.cpp",
     "capi.cpp",
+    "capi_io.cpp",
 ]
 
 # -ffp-contract=off: every float expression rounds as written (the projector
@@ -67,11 +74,19 @@ def build(verbose=True):
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("linking libetn.so failed:\n" + p.stderr)
     os.replace(tmp, OUT)
+    # the command-line tool: a plain C++ main over the C ABI, found next to the library
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "etn_main.cpp"),
+           "-o", tmp, "-L" + PKG, "-letn", "-Wl,-rpath,$ORIGIN"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("linking the etn tool failed:\n" + p.stderr)
+    os.replace(tmp, CLI)
     if verbose:
         print("built", OUT)
     return OUT

@@ -199,7 +199,15 @@ void BlockIterativeReconstructionOperator::handleOneIteration()
         doForwardProjectionForBlock(block);
         computeResidualsForBlock(block);
         doBackProjectionForBlock(block);
+        if (on_block_end_) on_block_end_(block);
     }
+}
+
+void BlockIterativeReconstructionOperator::setBlockEndCallback(std::function<void(uint32_t block)> callback)
+{
+    if (callback && (fused_ || use_graph_))
+        fail("a block-end callback needs the generic kernels launched eagerly");
+    on_block_end_ = std::move(callback);
 }
 
 void BlockIterativeReconstructionOperator::doForwardProjectionForBlock(uint32_t block)

@@ -2,6 +2,7 @@
 // (ettention/libReconstruction/src/algorithm/reconstruction/blockiterative/).
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <set>
 #include <string>
@@ -42,6 +43,11 @@ public:
     void tile(int* planes, int* band) const;
     // The launches of one iteration on stream(), without graph capture.
     void enqueueIteration();
+    // Called on the host after each block's back projection has been enqueued
+    // (the logger's OnBlockEnd hook, BlockIterativeReconstructionAlgorithmLogger.cpp:99-110).
+    // Needs the generic kernels launched eagerly (ETN_RECON_FLAG_GENERIC_KERNELS |
+    // ETN_RECON_FLAG_NO_GRAPH): the fused kernels overlap consecutive blocks.
+    void setBlockEndCallback(std::function<void(uint32_t block)> callback);
 
     // BlockIterativeProjectionAccess (BlockIterativeProjectionAccess.cpp:11-110)
     uint32_t GetBlockCount() const { return block_count_; }
@@ -90,6 +96,7 @@ private:
 
     bool use_graph_;
     bool tuned_ = true;
+    std::function<void(uint32_t)> on_block_end_;
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

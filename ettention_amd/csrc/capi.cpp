@@ -50,6 +50,14 @@ hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
 
+namespace etn {
+etn_status record_error(etn_status s, const std::string& message)
+{
+    g_last_error = message;
+    return s;
+}
+}  // namespace etn
+
 struct etn_reconstruction {
     std::unique_ptr<etn::BlockIterativeReconstructionOperator> op;
 };

@@ -22,6 +22,9 @@ struct Error : std::runtime_error {
     Error(etn_status s, const std::string& m) : std::runtime_error(m), status(s) {}
 };
 
+// Stores the message etn_last_error() returns on this thread; returns s.
+etn_status record_error(etn_status s, const std::string& message);
+
 [[noreturn]] inline void fail(const std::string& m) { throw Error(ETN_ERROR_INVALID_ARGUMENT, m); }
 [[noreturn]] inline void unsupported(const std::string& m) { throw Error(ETN_ERROR_UNSUPPORTED, m); }
 

@@ -287,6 +287,65 @@ etn_status etn_unet_train_step(etn_unet* n, const float* x, const float* d, cons
 /* HIP-event time of one forward pass and its convolution flops. */
 etn_status etn_unet_time_forward(etn_unet* n, const float* x, float scale, double* ms, double* flops);
 
+/* ---------------------------------------------------------------------------
+ * Input / output formats and the command-line reconstruction.  Host memory
+ * only; every array is caller-owned.
+ * ------------------------------------------------------------------------- */
+
+typedef struct etn_projection_stack etn_projection_stack;
+
+/* ImageStackDatasourceFactory::instantiate (io/ImageStackDataSourceFactory.cpp):
+ * a directory holding metadata.xml (ImageStackDirectoryDatasource.cpp:181-276)
+ * or an .mrc / .ali stack (MRCStack.cpp:136-263, modes 0/1/2/6, optional
+ * log(dataMax / (raw - min)) when `logarithmize`).  `tilt_angles` (a .tlt file,
+ * ImageStackDatasource.cpp:40-73) may be NULL or "" for an MRC stack (all
+ * angles 0) and is not used for a directory. */
+etn_status etn_projection_stack_open(const char* projections, const char* tilt_angles, int logarithmize,
+                                     etn_projection_stack** out);
+etn_status etn_projection_stack_destroy(etn_projection_stack* s);
+/* Detector size, projection count and the untilted base geometry. */
+etn_status etn_projection_stack_info(const etn_projection_stack* s, uint32_t* nu, uint32_t* nv, uint32_t* count,
+                                     etn_scanner_geometry* base);
+/* Copies the images ([count][nv][nu], stack order) and tilt angles (degrees);
+ * either pointer may be NULL. */
+etn_status etn_projection_stack_read(const etn_projection_stack* s, float* projections, float* tilt_angles);
+/* ImageStackDirectoryDatasource::writeMetaDataFile
+ * (ImageStackDirectoryDatasource.cpp:134-178) plus one float TIFF per
+ * projection, projection_<i>.tif, into an existing directory. */
+etn_status etn_image_stack_directory_write(const char* directory, const float* projections, uint32_t count,
+                                           uint32_t nu, uint32_t nv, const float* tilt_angles,
+                                           const etn_scanner_geometry* base, const uint32_t volume_resolution[3]);
+
+/* ImageDeserializer::readImage / readImageFromStack (ImageDeserializer.cpp):
+ * grey TIFF (float32, uint8 / 255, uint16 / 65535), row 0 of the file is row
+ * y = 0.  With pixels == NULL only the size is returned. */
+etn_status etn_tiff_read(const char* path, uint32_t page, uint32_t* width, uint32_t* height, float* pixels);
+etn_status etn_tiff_page_count(const char* path, uint32_t* pages);
+/* Uncompressed little-endian float32 TIFF. */
+etn_status etn_tiff_write(const char* path, const float* pixels, uint32_t width, uint32_t height);
+
+/* VolumeFileFormatWriter::write + MRCWriter (VolumeFileFormatWriter.cpp:11-144,
+ * MRCWriter.cpp:43-80).  `volume` is [z][y][x] with resolution (nx, ny, nz);
+ * voxel_type "unsigned8" | "signed16" | "unsigned16" | "float32"; orientation
+ * "xzy_order" (the reference default) | "xyz_order" | "yxz_order" |
+ * "yzx_order" | "zxy_order" | "zyx_order". */
+etn_status etn_mrc_write_volume(const char* path, const float* volume, const uint32_t resolution[3],
+                                const char* voxel_type, const char* orientation, int invert);
+/* VolumeDeserializer::load + applyInitialVolumeToWorkingVolume
+ * (BlockIterativeConfiguration.cpp:156-196): the file's axes mapped by
+ * `orientation`.  With volume == NULL only the resolution is returned. */
+etn_status etn_mrc_read_volume(const char* path, const char* orientation, uint32_t resolution[3], float* volume);
+
+/* The parameter sets a command line (and the --config XML file it names)
+ * resolves to, as one JSON object; *length receives the size without the
+ * terminating zero (CascadingParameterSource + AlgebraicParameterSet). */
+etn_status etn_config_resolve(int argc, const char* const* argv, char* json, size_t capacity, size_t* length);
+/* main() of the command-line tool (frontend/src/main.cpp:71-140): parses
+ * argv, reconstructs on the GPU and writes the output volume; returns the
+ * process exit code (0 success, 1 error, 2 unknown failure).  Progress and
+ * errors go to stdout. */
+int etn_main(int argc, const char* const* argv);
+
 #ifdef __cplusplus
 }
 #endif
