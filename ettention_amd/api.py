@@ -140,6 +140,13 @@ def compute_ray_length(vol_geom: VolumeGeometry, g: ScannerGeometry, out, stream
     call("etn_compute_ray_length", C.byref(vol_geom), C.byref(g), _ptr(out), _stream(stream))
 
 
+def ramp_filter(projection, out, stream=None):
+    """ConvolutionOperator with WBP's 128-tap RamLak kernel (etn_ramp_filter);
+    float32 device tensors of shape (nv, nu)."""
+    nv, nu = projection.shape[-2], projection.shape[-1]
+    call("etn_ramp_filter", int(nu), int(nv), _ptr(projection), _ptr(out), _stream(stream))
+
+
 def sum_squared_difference(a, b, out, stream=None):
     call("etn_sum_squared_difference_f64", _ptr(a), _ptr(b), a.numel(), _ptr(out), _stream(stream))
 
@@ -288,6 +295,16 @@ class BlockIterativeReconstructionOperator:
         torch.cuda.synchronize()
         call("etn_reconstruction_set_volume", self._h, _ptr(tensor))
         self.synchronize()
+
+
+class WeightedBackProjection(BlockIterativeReconstructionOperator):
+    """The WBP plugin's algorithm (etn_reconstruction_* with algorithm "wbp"):
+    run() filters and back-projects every projection once with lambda / N;
+    run_iterations(n) does projections 0..n-1."""
+
+    def __init__(self, **kw):
+        kw["algorithm"] = "wbp"
+        super().__init__(**kw)
 
 
 class BatchReconstruction:

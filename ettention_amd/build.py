@@ -24,11 +24,13 @@ SOURCES = [
     "projectors.hip",
     "fused.hip",
     "unet_kernels.hip",
+    "ramp_filter.hip",
     "geometry.cpp",
     "projection_set.cpp",
     "operators.cpp",
     "block_iterative.cpp",
     "batch.cpp",
+    "wbp.cpp",
     "unet.cpp",
     "xml.cpp",
     "io.cpp",

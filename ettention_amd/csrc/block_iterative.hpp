@@ -10,37 +10,38 @@
 
 #include "operators.hpp"
 #include "projection_set.hpp"
+#include "reconstruction.hpp"
 
 namespace etn {
 
 class FusedPlaneEngine;
 
-class BlockIterativeReconstructionOperator {
+class BlockIterativeReconstructionOperator final : public ReconstructionAlgorithm {
 public:
     BlockIterativeReconstructionOperator(const etn_reconstruction_desc& desc, hipStream_t stream);
-    ~BlockIterativeReconstructionOperator();
+    ~BlockIterativeReconstructionOperator() override;
 
     // ReconstructionAlgorithm (ReconstructionAlgorithm.h:9-22)
-    void run();
-    void runSingleIteration();
-    void runNiterations(uint32_t iterations);
-    GpuVolume* getReconstructedVolume() { return volume_.get(); }
+    void run() override;
+    void runSingleIteration() override;
+    void runNiterations(uint32_t iterations) override;
+    GpuVolume* getReconstructedVolume() override { return volume_.get(); }
     void setMaskVolume(const uint8_t* mask) { mask_.data = mask; invalidate_graph(); }
     const uint8_t* getMaskVolume() const { return mask_.data; }
 
     // Extension: ||A x - b||_2 over the used projections, FP64 accumulation (synchronises).
-    double residualNorm();
-    void synchronize();
+    double residualNorm() override;
+    void synchronize() override;
     // One eager iteration with HIP events around every launch; returns the
     // average milliseconds of the dominant kernel (0 fused, 1 FP, 2 BP).
-    double timeKernels(int* kernel, uint32_t* launches);
-    hipStream_t stream() const { return stream_; }
-    bool usesFusedPath() const { return fused_ != nullptr; }
+    double timeKernels(int* kernel, uint32_t* launches) override;
+    hipStream_t stream() const override { return stream_; }
+    bool usesFusedPath() const override { return fused_ != nullptr; }
     void invalidate_graph();
     // Fused path only: runs the tile autotuner once (unless disabled or a tile was set).
-    void ensureTuned();
-    void setTile(int planes, int band);
-    void tile(int* planes, int* band) const;
+    void ensureTuned() override;
+    void setTile(int planes, int band) override;
+    void tile(int* planes, int* band) const override;
     // The launches of one iteration on stream(), without graph capture.
     void enqueueIteration();
     // Called on the host after each block's back projection has been enqueued

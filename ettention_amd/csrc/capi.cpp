@@ -1,5 +1,6 @@
 // extern "C" entry points of libetn.so (include/etn/etn.h).  Exceptions of the
 // C++ layer are caught here and turned into etn_status + etn_last_error().
+#include <cctype>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -9,6 +10,7 @@
 #include "block_iterative.hpp"
 #include "etn/etn.h"
 #include "unet.hpp"
+#include "wbp.hpp"
 
 namespace {
 
@@ -59,7 +61,7 @@ etn_status record_error(etn_status s, const std::string& message)
 }  // namespace etn
 
 struct etn_reconstruction {
-    std::unique_ptr<etn::BlockIterativeReconstructionOperator> op;
+    std::unique_ptr<etn::ReconstructionAlgorithm> op;
 };
 
 struct etn_batch {
@@ -284,7 +286,12 @@ etn_status etn_reconstruction_create(const etn_reconstruction_desc* desc, void* 
         need(out, "output handle");
         *out = nullptr;
         auto r = std::make_unique<etn_reconstruction>();
-        r->op = std::make_unique<etn::BlockIterativeReconstructionOperator>(*desc, as_stream(stream));
+        // Framework::instantiateReconstructionAlgorithm (Framework.cpp:142-152): the
+        // block-iterative ids, else the plugin's "wbp" (WBPPlugin.cpp:21-24)
+        std::string id = desc->algorithm ? desc->algorithm : "";
+        for (char& c : id) c = (char)std::tolower((unsigned char)c);
+        if (id == "wbp") r->op = std::make_unique<etn::WeightedBackProjection>(*desc, as_stream(stream));
+        else r->op = std::make_unique<etn::BlockIterativeReconstructionOperator>(*desc, as_stream(stream));
         *out = r.release();
     });
 }
@@ -567,6 +574,21 @@ etn_status etn_unet_time_forward(etn_unet* n, const float* x, float scale, doubl
         need(x, "input volume");
         need(ms, "ms");
         *ms = n->net->timeForward(x, scale, flops);
+    });
+}
+
+etn_status etn_ramp_filter(uint32_t nu, uint32_t nv, const float* projection, float* filtered, void* stream)
+{
+    return guarded([&] {
+        need(projection, "projection");
+        need(filtered, "output");
+        if (!nu || !nv) etn::fail("empty image");
+        const auto p = etn::WeightedBackProjection::filterParams(nu, nv, etn::WeightedBackProjection::kFilterSize);
+        const std::vector<float> k = etn::WeightedBackProjection::ramLakKernel(etn::WeightedBackProjection::kFilterSize);
+        etn::DeviceBuffer taps(k.size() * sizeof(float));
+        ETN_HIP(hipMemcpyAsync(taps.get(), k.data(), taps.bytes(), hipMemcpyHostToDevice, as_stream(stream)));
+        etn::dev::launch_ramp_filter(p, taps.as<float>(), projection, filtered, as_stream(stream));
+        ETN_HIP(hipStreamSynchronize(as_stream(stream)));   // taps are freed on return
     });
 }
 

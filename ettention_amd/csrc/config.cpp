@@ -257,9 +257,9 @@ ReconstructionConfig ReconstructionConfig::from(const ParameterSource& s)
     else if (s.exists("algorithm.blockSize")) c.block_size = get_uint(s, "algorithm.blockSize");
     std::string lower = c.algorithm;
     std::transform(lower.begin(), lower.end(), lower.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
-    if (lower != "sart" && lower != "sirt" && lower != "blockiterative")
+    if (lower != "sart" && lower != "sirt" && lower != "blockiterative" && lower != "wbp")
         unsupported("reconstruction algorithm '" + c.algorithm +
-                    "' is not available (this library provides sart, sirt and blockIterative)");
+                    "' is not available (this library provides sart, sirt, blockIterative and wbp)");
     if (c.block_size == 0) fail("Illegal ProjectionBlockSize!");
     c.iterations = get_uint(s, "algorithm.numberOfIterations");
     c.lambda = get_float(s, "algorithm.lambda");

@@ -3,6 +3,7 @@
 
 #include <sys/stat.h>
 
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include <sstream>
 
 #include "block_iterative.hpp"
+#include "wbp.hpp"
 #include "common.hpp"
 
 namespace etn {
@@ -105,7 +107,7 @@ etn_voxel_type voxel_type_of(const std::string& r)
 }
 
 // The working volume as float on the host, [z][y][x].
-std::vector<float> download_volume(BlockIterativeReconstructionOperator& op)
+std::vector<float> download_volume(ReconstructionAlgorithm& op)
 {
     GpuVolume* v = op.getReconstructedVolume();
     op.synchronize();
@@ -175,6 +177,9 @@ void run_reconstruction(const ReconstructionConfig& c, std::ostream& log)
         ETN_HIP(hipMemcpy(mask.get(), bytes.data(), voxels, hipMemcpyHostToDevice));
     }
 
+    std::string id = c.algorithm;
+    for (char& ch : id) ch = (char)std::tolower((unsigned char)ch);
+    const bool wbp = id == "wbp";
     etn_reconstruction_desc d{};
     d.algorithm = "blockiterative";   // the block size is already resolved (AlgorithmParameterSet)
     d.block_size = c.block_size;
@@ -186,7 +191,8 @@ void run_reconstruction(const ReconstructionConfig& c, std::ostream& log)
     d.projection_iteration = c.projection_iteration.c_str();
     d.skip_projections = c.skip_projections.empty() ? nullptr : c.skip_projections.data();
     d.skip_count = (uint32_t)c.skip_projections.size();
-    d.voxel_type = voxel_type_of(c.voxel_representation);
+    // WBP always works on a FloatVolume (WBP.cpp:20)
+    d.voxel_type = wbp ? ETN_VOXEL_FLOAT32 : voxel_type_of(c.voxel_representation);
     for (int k = 0; k < 3; ++k) {
         const float half = 0.5f * c.dimension[k];   // VolumeParameterSet::getBoundingBox
         d.volume.resolution[k] = res[k];
@@ -203,7 +209,16 @@ void run_reconstruction(const ReconstructionConfig& c, std::ostream& log)
     d.mask = mask.as<uint8_t>();
     d.flags = c.write_intermediate_volumes ? (ETN_RECON_FLAG_GENERIC_KERNELS | ETN_RECON_FLAG_NO_GRAPH) : 0u;
 
-    BlockIterativeReconstructionOperator op(d, nullptr);
+    std::unique_ptr<ReconstructionAlgorithm> algorithm;
+    BlockIterativeReconstructionOperator* block_iterative = nullptr;
+    if (wbp) {
+        algorithm = std::make_unique<WeightedBackProjection>(d, nullptr);
+    } else {
+        auto bi = std::make_unique<BlockIterativeReconstructionOperator>(d, nullptr);
+        block_iterative = bi.get();
+        algorithm = std::move(bi);
+    }
+    ReconstructionAlgorithm& op = *algorithm;
 
     if (!c.init_file.empty()) {
         // VolumeDeserializer::load + applyInitialVolumeToWorkingVolume; a half
@@ -236,8 +251,8 @@ void run_reconstruction(const ReconstructionConfig& c, std::ostream& log)
         write_mrc_volume(join_path(c.debug_info_path, name), v.data(), res, VoxelValueType::Float32,
                          CoordinateOrder::XYZ, false);
     };
-    if (c.write_intermediate_volumes) {
-        op.setBlockEndCallback([&](uint32_t block) {
+    if (c.write_intermediate_volumes && block_iterative) {
+        block_iterative->setBlockEndCallback([&](uint32_t block) {
             char name[64];
             std::snprintf(name, sizeof name, "intermediate_i%03u_b%03u.mrc", iteration, block);
             dump(name);
@@ -246,7 +261,12 @@ void run_reconstruction(const ReconstructionConfig& c, std::ostream& log)
 
     log << "Launching reconstruction algorithm...\n";
     const auto t0 = std::chrono::steady_clock::now();
-    for (iteration = 0; iteration < c.iterations; ++iteration) {
+    if (wbp) {
+        // WBP::run filters and back-projects every projection once; the
+        // iteration count and the intermediate volumes do not apply
+        op.run();
+    }
+    for (iteration = 0; !wbp && iteration < c.iterations; ++iteration) {
         const auto ti = std::chrono::steady_clock::now();
         op.runNiterations(1);
         if (prof >= 2) {

@@ -50,6 +50,24 @@ void launch_sum_squared_difference(const float* a, const float* b, size_t count,
                                    hipStream_t stream);
 void launch_convert(const void* src, etn_voxel_type st, void* dst, etn_voxel_type dt, size_t count, hipStream_t stream);
 void launch_fill(void* dst, etn_voxel_type t, float value, size_t count, hipStream_t stream);
+
+// Ramp filtering of one projection as the reference's Fourier-space
+// convolution computes it (ConvolutionFourierSpaceImplementation.cpp:50-83):
+// the image mirror-padded to ux x uy (powers of two), convolved circularly
+// with `taps` filter values centred on tap taps / 2, the filter one padded row
+// above the origin, cropped back to nu x nv.
+struct RampFilterParams {
+    uint32_t nu, nv;            // image
+    uint32_t ux, uy;            // padded size
+    int32_t pad_x, pad_y;       // first image column / row inside the padded image
+    int32_t end_x, end_y;       // one past the last
+    int32_t crop_x, crop_y;     // first padded column / row of the cropped result
+    uint32_t taps;
+};
+constexpr uint32_t kMaxRampRow = 8192;    // padded row length held in LDS
+constexpr uint32_t kMaxRampTaps = 512;
+void launch_ramp_filter(const RampFilterParams& p, const float* taps, const float* src, float* dst,
+                        hipStream_t stream);
 void launch_zero(float* dst, size_t count, hipStream_t stream);
 
 }  // namespace dev

@@ -1,0 +1,127 @@
+#include "wbp.hpp"
+
+#include <algorithm>
+
+#include "geometry.hpp"
+
+namespace etn {
+
+namespace {
+
+// NumericalAlgorithms::nearestPowerOfTwo (NumericalAlgorithms.cpp:20-31)
+uint32_t next_power_of_two(uint32_t v)
+{
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1;
+}
+
+}  // namespace
+
+std::vector<float> WeightedBackProjection::ramLakKernel(uint32_t resolution)
+{
+    const float pi = 3.14159265358979323846f;
+    const float pi2 = pi * pi;   // M_PI_F * M_PI_F, a float product
+    std::vector<float> k(resolution);
+    for (uint32_t i = 0; i < resolution; ++i) {
+        const int position = (int)i - (int)(resolution / 2);
+        if (position == 0) k[i] = 1.0f / 4.0f;
+        else if (position % 2 == 0) k[i] = 0.0f;
+        else k[i] = -1.0f / ((float)(position * position) * pi2);
+    }
+    return k;
+}
+
+dev::RampFilterParams WeightedBackProjection::filterParams(uint32_t nu, uint32_t nv, uint32_t taps)
+{
+    if (taps == 0 || taps % 2) fail("the ramp filter needs an even, non-zero number of taps");
+    dev::RampFilterParams p{};
+    p.nu = nu;
+    p.nv = nv;
+    // ConvolutionFourierSpaceImplementation::getUnifiedResolution: the larger
+    // of filter (taps x 1) and image, rounded up to powers of two
+    p.ux = next_power_of_two(std::max(taps, nu));
+    p.uy = next_power_of_two(std::max(1u, nv));
+    p.pad_x = ((int32_t)p.ux - (int32_t)nu) / 2;
+    p.pad_y = ((int32_t)p.uy - (int32_t)nv) / 2;
+    p.end_x = ((int32_t)p.ux + (int32_t)nu) / 2;
+    p.end_y = ((int32_t)p.uy + (int32_t)nv) / 2;
+    p.crop_x = (int32_t)(p.ux / 2) - (int32_t)(nu / 2);
+    p.crop_y = (int32_t)(p.uy / 2) - (int32_t)(nv / 2);
+    p.taps = taps;
+    if (p.ux > dev::kMaxRampRow)
+        unsupported("ramp filter: detector rows longer than " + std::to_string(dev::kMaxRampRow / 2) +
+                    " pixels are not supported");
+    return p;
+}
+
+WeightedBackProjection::WeightedBackProjection(const etn_reconstruction_desc& d, hipStream_t stream)
+    : count_(d.projection_count), lambda_(d.lambda), projections_(d.projections)
+{
+    if (d.projection_count == 0 || !d.tilt_angles) fail("No projections given!");
+    if (!d.projections) fail("projection stack device pointer is null");
+    const uint32_t nu = d.base_geometry.resolution[0], nv = d.base_geometry.resolution[1];
+    if (nu == 0 || nv == 0) fail("detector resolution is zero");
+    pixels_ = (size_t)nu * nv;
+    params_ = filterParams(nu, nv, kFilterSize);
+
+    const SingleAxisTiltRotator rotator{ParallelScannerGeometry(d.base_geometry)};
+    geometries_.reserve(count_);
+    for (uint32_t i = 0; i < count_; ++i) geometries_.push_back(rotator.rotated(d.tilt_angles[i], d.x_axis_tilt));
+
+    if (stream) {
+        stream_ = stream;
+    } else {
+        ETN_HIP(hipStreamCreate(&stream_));
+        own_stream_ = true;
+    }
+    etn_volume_geometry vg = d.volume;
+    if (vg.subvolume_count == 0) vg.subvolume_count = 1;
+    volume_ = std::make_unique<GpuVolume>(vg, d.voxel_type);
+    volume_->fill(d.init_value, stream_);
+    setup_ = std::make_unique<GeometricSetup>(geometries_[0], &volume_->properties());
+    backward_ = std::make_unique<BackProjectionOperator>(setup_.get(), volume_.get(), &mask_, lambda_,
+                                                         d.bp_samples ? d.bp_samples : 1, stream_);
+    images_.allocate(2 * pixels_ * sizeof(float));
+    filtered_ = {images_.as<float>(), nu, nv};
+    length_ = {images_.as<float>() + pixels_, nu, nv};
+    ray_length_ = std::make_unique<ComputeRayLengthKernel>(setup_.get(), &length_, stream_);
+    const std::vector<float> k = ramLakKernel(kFilterSize);
+    taps_.allocate(k.size() * sizeof(float));
+    ETN_HIP(hipMemcpyAsync(taps_.get(), k.data(), taps_.bytes(), hipMemcpyHostToDevice, stream_));
+    ETN_HIP(hipStreamSynchronize(stream_));
+}
+
+WeightedBackProjection::~WeightedBackProjection()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void WeightedBackProjection::filter(const float* src, float* dst)
+{
+    dev::launch_ramp_filter(params_, taps_.as<float>(), src, dst, stream_);
+}
+
+void WeightedBackProjection::run() { runNiterations(count_); }
+
+void WeightedBackProjection::runNiterations(uint32_t n)
+{
+    if (n > count_) fail("WBP: more projections requested than the stack holds");
+    for (uint32_t i = 0; i < n; ++i) {
+        setup_->setScannerGeometry(geometries_[i]);
+        filter(projections_ + (size_t)i * pixels_, filtered_.data);
+        ray_length_->run();
+        backward_->setInput(&filtered_, &length_, count_);
+        backward_->execute();
+        backward_->setInput(nullptr, nullptr, 1);
+    }
+}
+
+void WeightedBackProjection::synchronize() { ETN_HIP(hipStreamSynchronize(stream_)); }
+
+}  // namespace etn

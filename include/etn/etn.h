@@ -139,6 +139,12 @@ etn_status etn_long_object_compensate(uint32_t nu, uint32_t nv, float* projectio
  * bounding box (ComputeRayLengthKernel.cpp:9-46; ocl_src/ComputeRayLength.cl). */
 etn_status etn_compute_ray_length(const etn_volume_geometry* vol, const etn_scanner_geometry* g,
                                   float* ray_length, void* stream);
+/* ConvolutionOperator with the 128-tap RamLak kernel as WBP sets it up
+ * (WBP.cpp:29-33; GenerateRamLakKernel.cl; ConvolutionFourierSpaceImplementation.cpp):
+ * the image mirror-padded to powers of two, convolved circularly along rows
+ * with the filter placed one padded row above the origin, cropped back.
+ * Device images of nu x nv floats; synchronous on `stream`. */
+etn_status etn_ramp_filter(uint32_t nu, uint32_t nv, const float* projection, float* filtered, void* stream);
 
 /* Extension (no reference counterpart; the reference's only metric is the racy
  * BufferRMS.cl): out_device[0] += sum_i (a_i - b_i)^2 accumulated in FP64, with b
@@ -171,7 +177,7 @@ enum {
  * ForwardProjectionParameterSet, BackprojectionParameterSet, VolumeParameterSet,
  * InputParameterSet, HardwareParameterSet). */
 typedef struct etn_reconstruction_desc {
-    const char* algorithm;              /* "sart" | "sirt" | "blockiterative" (case-insensitive) */
+    const char* algorithm;              /* "sart" | "sirt" | "blockiterative" | "wbp" (case-insensitive) */
     uint32_t block_size;                /* algorithm.blockSize (blockiterative only) */
     uint32_t iterations;                /* algorithm.numberOfIterations (run) */
     float lambda;                       /* algorithm.lambda */
@@ -193,6 +199,14 @@ typedef struct etn_reconstruction_desc {
     uint32_t flags;                     /* ETN_RECON_FLAG_* */
 } etn_reconstruction_desc;
 
+/* The block-iterative ids select BlockIterativeReconstructionOperator
+ * (Framework.cpp:142-152).  "wbp" selects weighted back projection
+ * (plugins/WBPPlugin/src/reconstruction/WBP.cpp:13-84), which reads
+ * lambda, bp_samples, volume, init_value, voxel_type, the stack, its tilt
+ * angles and x_axis_tilt; run() filters and back-projects every projection
+ * once, run_iterations(n) projections 0..n-1 (WBP::runNiterations).  The
+ * block-iterative extensions (residual norm, kernel timing, tiles) return
+ * ETN_ERROR_UNSUPPORTED for it. */
 etn_status etn_reconstruction_create(const etn_reconstruction_desc* desc, void* stream,
                                      etn_reconstruction** out);
 etn_status etn_reconstruction_destroy(etn_reconstruction* r);
