@@ -115,6 +115,9 @@ struct PlaneRay {
     float stepx, stepz;
     int sx, sz;
     int off;                     // LDS offset of the ray's plane inside a band row
+    // without the step cap the band loop tracks the ray by its LDS byte offset
+    // in the current band and by the x steps left before it leaves the volume
+    int li, cx, dxb, dzb;
 };
 
 // Advances a ray while it is inside the volume and inside band [zlo, zhi).  A
@@ -128,37 +131,44 @@ template <bool kCap, bool kFast>
 __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int width, int nx, int nz, int zlo,
                                               int zhi)
 {
-    if (!r.live) return;
-    const unsigned h = (unsigned)(zhi - zlo);
-    const int lz = r.sz * width;
-    int li = (r.vz - zlo) * width + r.off + r.vx;
-    int vx = r.vx, vz = r.vz;
-    float rx = r.rx, rz = r.rz, acc = r.acc, accl = r.accl;
     if (!kCap) {
-        // one exit condition; whether the ray left the volume or only the band
-        // is decided after the loop (the same steps as the loop below)
-        while ((unsigned)vx < (unsigned)nx && (unsigned)(vz - zlo) < h) {
+        // r.li is the byte offset of the ray's voxel in this band's LDS rows:
+        // the voxel lies in the band exactly when 0 <= li < band bytes (its x
+        // index is in range while x steps are left, so the row part stays
+        // inside the ray's plane).  r.cx counts the x steps left inside the
+        // volume.  The float operations are the loop below's: the step not
+        // taken loses dt, |rx - rz| being exactly max - min.
+        const unsigned lim = (unsigned)((zhi - zlo) * width) * 4u;
+        int li = r.li, cx = r.cx;
+        float rx = r.rx, rz = r.rz, acc = r.acc, accl = r.accl;
+        const char* bytes = reinterpret_cast<const char*>(band);
+        while (cx != 0 && (unsigned)li < lim) {
             const bool ix = rx <= rz;
             const float dt = ix ? rx : rz;
-            if (kFast) acc = fmaf(dt, band[li], acc);
-            else acc += dt * band[li];
+            const float value = *reinterpret_cast<const float*>(bytes + li);
+            if (kFast) acc = fmaf(dt, value, acc);
+            else acc += dt * value;
             accl += dt;
-            const float rest = (ix ? rz : rx) - dt;
+            const float rest = fabsf(rx - rz);
             rx = ix ? r.stepx : rest;
             rz = ix ? rest : r.stepz;
-            vx += ix ? r.sx : 0;
-            vz += ix ? 0 : r.sz;
-            li += ix ? r.sx : lz;
+            li += ix ? r.dxb : r.dzb;
+            cx -= ix ? 1 : 0;
         }
-        r.live = ((unsigned)vx < (unsigned)nx && (unsigned)vz < (unsigned)nz) ? 1 : 0;
-        r.vx = vx;
-        r.vz = vz;
+        r.li = li;
+        r.cx = cx;
         r.rx = rx;
         r.rz = rz;
         r.acc = acc;
         r.accl = accl;
         return;
     }
+    if (!r.live) return;
+    const unsigned h = (unsigned)(zhi - zlo);
+    const int lz = r.sz * width;
+    int li = (r.vz - zlo) * width + r.off + r.vx;
+    int vx = r.vx, vz = r.vz;
+    float rx = r.rx, rz = r.rz, acc = r.acc, accl = r.accl;
     for (;;) {
         const bool inx = (unsigned)vx < (unsigned)nx;
         if (!(inx && (unsigned)(vz - zlo) < h)) {
@@ -439,6 +449,16 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
 
     int z0, z1;
     band_range(0, z0, z1);
+    if (!kCap) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            PlaneRay& r = ray[k];
+            r.li = ((r.vz - z0) * width + r.off + r.vx) * 4;
+            r.cx = !r.live ? 0 : r.sx > 0 ? nx - r.vx : r.sx < 0 ? r.vx + 1 : 0x7fffffff;
+            r.dxb = 4 * r.sx;
+            r.dzb = 4 * r.sz * width;
+        }
+    }
     prefetch(z0, z1);
     if (nbp == 1) stage_rows(0);     // made visible by the first band's barrier
 
@@ -590,6 +610,14 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
             for (int k = 0; k < RPT; ++k) {
                 const int id = k * kFusedThreads + tid;
                 if (id < nrays) plane_advance<kCap, kFast>(ray[k], band, width, nx, nz, z0, z1);
+            }
+            if (!kCap && bi + 1 < nbands) {
+                // re-base the LDS offsets on the next band's first row
+                int n0, n1;
+                band_range(bi + 1, n0, n1);
+                const int shift = (z0 - n0) * width * 4;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) ray[k].li += shift;
             }
         }
         __syncthreads();
