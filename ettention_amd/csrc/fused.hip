@@ -290,7 +290,9 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     using IO = GroupIo<S, kVec>;
     constexpr int G = IO::G;
     constexpr int kGroups = kMaxBandFloats / (G * kFusedThreads);   // band groups per thread (upper bound)
-    extern __shared__ float lds[];
+    // a fixed-size array puts the band at LDS address 0 (the host sizes every
+    // tile to fit it; the VGPR budget already limits a CU to two workgroups)
+    __shared__ __attribute__((aligned(16))) float lds[kLdsFloats];
     const FusedWork item = work[blockIdx.x];
     const ETN_GLOBAL FusedVolume& fv = global_ptr(vols)[item.member];
     const ETN_GLOBAL FusedPass* pass = global_ptr(global_ptr(passes)[item.member]);
@@ -678,12 +680,13 @@ struct FusedLaunch {
 template <typename S, bool kVec, bool kCap, bool kFast>
 void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 {
+    if (l.lds > sizeof(float) * (size_t)kLdsFloats) fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kFusedThreads);
     switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, kFast, 1><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, kFast, 2><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, kFast, 4><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, kFast, 8><<<grid, block, l.lds, stream>>>(l.vols, l.work, l.passes); break;
+        case 1: fused_plane_kernel<S, kVec, kCap, kFast, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, kFast, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, kFast, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, kFast, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
     }
 }
 
