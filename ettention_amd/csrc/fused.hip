@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     for (int bi = 0; bi < nbands; ++bi) {
         band_range(bi, z0, z1);
         const int ng = (z1 - z0) * gpr;
-        // 1. prefetched band -> LDS, then start fetching the next band
+        // 1. prefetched band -> LDS
         if (kVec) {
 #pragma unroll
             for (int k = 0; k < kPre; ++k) {
@@ -489,12 +489,15 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 for (int i = 0; i < G; ++i) band[zz * width + xg * G + i] = v[i];
             }
         }
-        __syncthreads();
+        // the prefetch registers are free again: start fetching the next band
+        // before waiting for the other waves (it only reads planes this
+        // workgroup stores later)
         if (bi + 1 < nbands) {
             int n0, n1;
             band_range(bi + 1, n0, n1);
             prefetch(n0, n1);
         }
+        __syncthreads();
 
         // 2. back projections of the previous block, in block order; each thread
         //    updates (and finally stores) the groups it staged
