@@ -476,8 +476,8 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     float v[G];
                     IO::unpack(pre[k], v);
                     const int zz = gi / gpr, xg = gi - zz * gpr;
-#pragma unroll
-                    for (int i = 0; i < G; ++i) band[zz * width + xg * G + i] = v[i];
+                    // one 16-byte LDS store (width and xg * G are multiples of 4)
+                    *reinterpret_cast<float4*>(band + zz * width + xg * G) = make_float4(v[0], v[1], v[2], v[3]);
                 }
             }
         } else {
@@ -575,10 +575,21 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 for (int zz = tid / gpr; zz < nrow; zz += dz) {
                     const float zt = z_term(zz);
                     float out[G];
+                    float* cell = band + zz * width + xw;
+                    if constexpr (G == 4) {
+                        // four voxels, 16-byte aligned (width and xw are multiples of 4)
+                        const float4 in = *reinterpret_cast<const float4*>(cell);
+                        out[0] = update(pr, pt.y, xt[0], zt, in.x);
+                        out[1] = update(pr, pt.y, xt[1], zt, in.y);
+                        out[2] = update(pr, pt.y, xt[2], zt, in.z);
+                        out[3] = update(pr, pt.y, xt[3], zt, in.w);
+                        *reinterpret_cast<float4*>(cell) = make_float4(out[0], out[1], out[2], out[3]);
+                    } else {
 #pragma unroll
-                    for (int i = 0; i < G; ++i) {
-                        out[i] = update(pr, pt.y, xt[i], zt, band[zz * width + xw + i]);
-                        band[zz * width + xw + i] = out[i];
+                        for (int i = 0; i < G; ++i) {
+                            out[i] = update(pr, pt.y, xt[i], zt, cell[i]);
+                            cell[i] = out[i];
+                        }
                     }
                     if (q == nbp - 1)
                         IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
