@@ -811,13 +811,27 @@ std::vector<Tile> candidate_tiles(const MemberPlan& m)
     return out;
 }
 
+// The fused kernels sweep the whole volume.  A z-slab split
+// (hardware.subVolumeCount) restarts the reference's forward projection at
+// every slab face and sums the slab parts; sweeping the volume in one pass
+// takes the same steps and only regroups those sums and the voxel-centre
+// arithmetic, so it is allowed with fast arithmetic and left to the generic
+// kernels with exact arithmetic.
+VolumeProperties whole_volume(const VolumeProperties& vp)
+{
+    etn_volume_geometry g = vp.desc();
+    g.subvolume_count = 1;
+    return VolumeProperties(g);
+}
+
 // Eligibility of one reconstruction and its plane/row tables; false when it
 // has no fused form.
 bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d, bool exact,
                  MemberPlan& out)
 {
-    const VolumeProperties& vp = op.getReconstructedVolume()->properties();
-    if (vp.subvolume_count() != 1) return false;
+    if (exact && op.getReconstructedVolume()->properties().subvolume_count() != 1) return false;
+    const VolumeProperties whole = whole_volume(op.getReconstructedVolume()->properties());
+    const VolumeProperties& vp = whole;
     if ((d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1) return false;
     if (d.mask) return false;
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
@@ -900,7 +914,6 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     if (!exact) {
         // the fast lookup skips the column clamp: stage every column u - 0.5 can
         // floor to for a voxel centre of this member, plus one column either side
-        const VolumeProperties& vp = op.getReconstructedVolume()->properties();
         const Vec3f vs = vp.voxel_size(), lo = vp.bbox_min(), hi = vp.bbox_max();
         double umin = 1e300, umax = -1e300;
         for (uint32_t b = 0; b < op.GetBlockCount(); ++b)
@@ -950,7 +963,7 @@ void build_passes(MemberPlan& mb)
     auto image = [&](uint32_t parity, uint32_t slot, int which) {
         return mb.images.as<float>() + mb.pixels * ((((size_t)parity * mb.bmax + slot) * 2) + which);
     };
-    const VolumeProperties& vp = op.getReconstructedVolume()->properties();
+    const VolumeProperties vp = whole_volume(op.getReconstructedVolume()->properties());
     std::vector<dev::FusedBp> pending;   // back projections of the previous block
     auto flush_bp_only = [&](size_t keep) {
         // emit BP-only passes until at most `keep` back projections are pending
@@ -1077,7 +1090,7 @@ void FusedEngineImpl::upload(hipStream_t stream)
         pass_base_.push_back(all.size());
         all.insert(all.end(), m.passes.begin(), m.passes.end());
         GpuVolume* vol = m.op->getReconstructedVolume();
-        const VolumeProperties& vp = vol->properties();
+        const VolumeProperties vp = whole_volume(vol->properties());
         dev::FusedVolume fv{};
         fv.data = vol->data();
         fv.slab = make_slab_params(vp, 0);
