@@ -1,13 +1,15 @@
 """SART throughput on one MI355X: GUPS on the 512^3 x 121-tilt workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxel-type float|half|bfloat16]
-                    [--workload metric|c2|c4] [--generic] [--no-cpu-baseline]
+                    [--workload metric|c2|c3|c4|c5] [--generic] [--mask] [--no-cpu-baseline]
 
 A step is one full iteration of the block-iterative loop over the synthetic
 tilt series (for SART: every projection's forward projection, residual and
 back projection).  GUPS = voxels x projections x steps / seconds / 1e9, with
 the volume, the measured projections and every buffer resident in HBM when
 the timed region starts.  Prints ONE JSON line on stdout; progress on stderr.
+--mask runs the metric and c2 workloads with a prior-knowledge mask (weight 1
+inside a y-axis cylinder holding the specimen, 0 outside).
 """
 from __future__ import annotations
 
@@ -61,6 +63,7 @@ def main():
     ap.add_argument("--voxel-type", default="float", choices=["float", "half", "bfloat16"])
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--generic", action="store_true", help="force the generic (unfused) kernels")
+    ap.add_argument("--mask", action="store_true", help="metric/c2: reconstruct with a prior-knowledge mask")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-projections", type=int, default=32,
                     help="projection steps the CPU baseline times (about 0.35 s each on 16 threads)")
@@ -75,6 +78,9 @@ def main():
               "no exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.mask and args.workload not in ("metric", "c2"):
+        print("--mask applies to the metric and c2 workloads", file=sys.stderr)
+        sys.exit(2)
     if args.workload == "c4":
         run_batch(args)
         return
@@ -114,11 +120,12 @@ def main():
     log("synthetic input ready in %.1f s (phantom max %.2f)" % (time.time() - t0, float(phantom.max())))
     del phantom, scratch
 
+    mask = support_mask(nx, ny, nz, dev) if args.mask else None
     flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
     r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs,
                                                algorithm=wl["algorithm"], block_size=wl["block_size"],
                                                lam=wl["lam"], projection_iteration=wl["order"],
-                                               voxel_type=args.voxel_type, flags=flags)
+                                               voxel_type=args.voxel_type, flags=flags, mask=mask)
     fused = r.uses_fused_path()
     for _ in range(args.warmup):
         r.run_single_iteration()
@@ -139,7 +146,7 @@ def main():
     log("residual norm after %d iterations: %.6e" % (args.warmup + args.steps, norm))
 
     block = wl["block_size"] or (1 if wl["algorithm"] == "sart" else N)
-    roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block)
+    roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block, masked=mask is not None)
     del r
     torch.cuda.empty_cache()
 
@@ -153,7 +160,9 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, tilts, projs, args.cpu_sample_projections)
+        cpu = cpu_baseline(wl, tilts, projs, args.cpu_sample_projections,
+                           None if mask is None else mask.cpu().numpy())
+    del mask
 
     out = {
         "metric": "SART GUPS (proj+backproj) 512^3 x 121 tilts",
@@ -176,7 +185,22 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if args.mask:
+        out["config"]["mask"] = MASK_NAME
     print(json.dumps(out), flush=True)
+
+
+MASK_NAME = "prior-knowledge mask: 255 inside the y-axis cylinder of radius 0.45 nx centred in x-z, 0 outside"
+
+
+def support_mask(nx, ny, nz, dev):
+    """USE_PRIOR_KNOWLEDGE_MASK input for the benchmark: uint8 (nz, ny, nx), 255
+    inside a y-axis cylinder that holds the phantom's support, 0 outside."""
+    import torch
+    x = torch.arange(nx, device=dev, dtype=torch.float32) + 0.5 - nx / 2
+    z = torch.arange(nz, device=dev, dtype=torch.float32) + 0.5 - nz / 2
+    inside = (x[None, :] ** 2 + z[:, None] ** 2) <= (0.45 * nx) ** 2
+    return (inside.to(torch.uint8) * 255)[:, None, :].expand(nz, ny, nx).contiguous()
 
 
 def run_precision(args):
@@ -410,7 +434,7 @@ def run_learned(args):
     print(json.dumps(out), flush=True)
 
 
-def kernel_roofline(r, voxel_type, fused, V, P, block=1):
+def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False):
     """Average duration of the dominant kernel, measured with HIP events around
     every launch of one extra eager iteration on the reconstruction's stream,
     and its algorithmic bytes per launch.
@@ -420,24 +444,28 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1):
     algorithmic bytes are the per-block figure of the unfused schedule,
     3 s V + 20 P B (the FP reads the volume, the BP reads and writes it, 20
     bytes per detector pixel and projection).  Generic FP: s V + 8 P; generic
-    BP: 2 s V + 8 P."""
+    BP: 2 s V + 8 P.  A mask adds one byte per voxel to each volume read
+    (the FP and the BP each read it)."""
     s = 4 if voxel_type == "float" else 2
+    m = 1 if masked else 0
     kernel, dur_ms, launches = r.time_kernels()
     if kernel == 0:
         name = ("fused_plane_kernel (BP of projection k-1 + FP/residual of projection k)" if block == 1 else
                 "fused_plane_kernel (BP of block b-1 + FP/residual of block b, %d projections each)" % block)
-        alg_bytes = 3 * s * V + 20 * P * block
-        kernel_bytes = 2 * s * V + 28 * P * block
+        alg_bytes = (3 * s + 2 * m) * V + 20 * P * block
+        kernel_bytes = (2 * s + m) * V + 28 * P * block
     elif kernel == 1:
-        name, alg_bytes = "forward_project_kernel (generic DDA)", s * V + 8 * P
+        name, alg_bytes = "forward_project_kernel (generic DDA)", (s + m) * V + 8 * P
         kernel_bytes = alg_bytes
     else:
-        name, alg_bytes = "back_project_kernel (generic)", 2 * s * V + 8 * P
+        name, alg_bytes = "back_project_kernel (generic)", (2 * s + m) * V + 8 * P
         kernel_bytes = alg_bytes
+    if masked:
+        name += ", masked"
     achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
     log("dominant kernel %s: %.4f ms average over %d launches -> %.1f GB/s algorithmic"
         % (name, dur_ms, launches, achieved))
-    traffic, source, l2 = measured_traffic(kernel, voxel_type)
+    traffic, source, l2 = measured_traffic(kernel, voxel_type) if not masked else (None, None, None)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
             "l2_hit_rate": l2, "kernel": name,
@@ -463,11 +491,11 @@ def measured_traffic(kernel, voxel_type):
     return traffic, os.path.relpath(path, ROOT), l2
 
 
-def cpu_baseline(wl, tilts, projs, nsample):
+def cpu_baseline(wl, tilts, projs, nsample, mask=None):
     """The oracle (FP32-faithful C restatement of the reference's OpenCL path,
     OpenMP over the host cores) on a bounded sample of the same workload: the
     first `nsample` projection steps (FP + residual + BP) of a SART pass over
-    the full-size volume."""
+    the full-size volume, with the workload's mask if it has one."""
     from oracle import oracle as O
     nx, ny, nz = wl["res"]
     nu, nv = wl["det"]
@@ -478,15 +506,15 @@ def cpu_baseline(wl, tilts, projs, nsample):
     t0 = time.perf_counter()
     for i in range(nsample):
         g = O.rotate(base, float(tilts[i]))
-        p, l = O.forward_project(v, vol, g)
+        p, l = O.forward_project(v, vol, g, mask=mask)
         res = O.compare(meas[i], p)
-        O.back_project(v, vol, g, res, l, lam=wl["lam"])
+        O.back_project(v, vol, g, res, l, lam=wl["lam"], mask=mask)
     dt = time.perf_counter() - t0
     gups = nx * ny * nz * nsample / dt / 1e9
     log("cpu baseline: %d projection steps in %.1f s on %d threads -> %.4f GUPS" % (nsample, dt, O.num_threads(), gups))
     return {"value": round(gups, 5), "unit": "GUPS", "cores": O.num_threads(), "kind": "port",
-            "sample": "%d SART projection steps (FP+residual+BP) on the full %dx%dx%d volume, %d^2 detector"
-                      % (nsample, nx, ny, nz, nu)}
+            "sample": "%d SART projection steps (FP+residual+BP) on the full %dx%dx%d volume, %d^2 detector%s"
+                      % (nsample, nx, ny, nz, nu, ", masked" if mask is not None else "")}
 
 
 if __name__ == "__main__":

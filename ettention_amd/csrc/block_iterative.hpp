@@ -26,7 +26,14 @@ public:
     void runSingleIteration() override;
     void runNiterations(uint32_t iterations) override;
     GpuVolume* getReconstructedVolume() override { return volume_.get(); }
-    void setMaskVolume(const uint8_t* mask) { mask_.data = mask; invalidate_graph(); }
+    // The fused plane kernels take the mask given at creation; with them a
+    // different mask is refused (create the reconstruction with it instead).
+    void setMaskVolume(const uint8_t* mask)
+    {
+        if (fused_ && mask != mask_.data) unsupported("the fused plane kernels keep the mask given at creation");
+        mask_.data = mask;
+        invalidate_graph();
+    }
     const uint8_t* getMaskVolume() const { return mask_.data; }
 
     // Extension: ||A x - b||_2 over the used projections, FP64 accumulation (synchronises).

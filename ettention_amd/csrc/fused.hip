@@ -42,6 +42,9 @@ constexpr int kMaxPlanes = 8;        // planes per workgroup
 constexpr int kLdsFloats = 16384;    // 64 KiB per workgroup
 constexpr int kRowPad = 2;           // zero taps on each side of a staged detector row
 constexpr int kMaxBandFloats = 12288;   // band voxels: the vector path prefetches 6 groups of 4 per thread
+// With a prior-knowledge mask the band holds half as many voxels and the
+// voxels' mask weights follow at this fixed LDS offset (floats).
+constexpr int kMaskBandFloats = kMaxBandFloats / 2;
 
 struct FusedFp {
     float src[3], h[3], v[3], ray[3];
@@ -82,6 +85,7 @@ struct FusedVolume {
     const int* row_hi;           // per plane: one past the last such row
     const int* dead_rows;        // detector rows whose rays all miss the volume in y
     int ndead;
+    const uint8_t* mask;         // prior-knowledge mask (complete volume, 255 = weight 1) or null: weight 1
     int* diag;                   // counts live rays found outside their plane (must stay 0)
 };
 
@@ -90,12 +94,13 @@ struct FusedWork {
     int plane;                   // first plane of the workgroup's group
 };
 
-// LDS floats a workgroup uses: the band, then (16-byte aligned) the staged
+// LDS floats a workgroup uses: the band (with a mask: the band, at
+// kMaskBandFloats the band's mask weights), then (16-byte aligned) the staged
 // detector records of each plane (row_floats per plane: 8 floats per padded
 // column with exact arithmetic, 4 with fast) and a (m4 py, ty) pair per plane.
-__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats)
+__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats, bool mask)
 {
-    return ((band * planes * nx + 3) & ~3) + planes * (row_floats + 2);
+    return (mask ? kMaskBandFloats : 0) + ((band * planes * nx + 3) & ~3) + planes * (row_floats + 2);
 }
 
 namespace {
@@ -126,11 +131,16 @@ struct PlaneRay {
 // dropped when nx + nz <= 2049, since every step moves vx or vz monotonically by
 // one, so a ray can take at most nx + nz - 1 steps inside the volume.
 // kFast accumulates dt * value with a fused multiply-add (one rounding instead
-// of two, as an OpenCL compiler may contract it).
-template <bool kCap, bool kFast>
+// of two, as an OpenCL compiler may contract it).  kMask weights the ray
+// length by the voxel's mask weight, stored kMaskBandFloats after the voxel.
+template <bool kCap, bool kFast, bool kMask>
 __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int width, int nx, int nz, int zlo,
                                               int zhi)
 {
+    auto length_step = [](float accl, float dt, float mv) {
+        if (!kMask) return accl + dt;                   // dt * 1.0f
+        return kFast ? fmaf(dt, mv, accl) : accl + dt * mv;
+    };
     if (!kCap) {
         // r.li is the byte offset of the ray's voxel in this band's LDS rows:
         // the voxel lies in the band exactly when 0 <= li < band bytes (its x
@@ -148,7 +158,7 @@ __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, in
             const float value = *reinterpret_cast<const float*>(bytes + li);
             if (kFast) acc = fmaf(dt, value, acc);
             else acc += dt * value;
-            accl += dt;
+            accl = length_step(accl, dt, kMask ? *reinterpret_cast<const float*>(bytes + li + 4 * kMaskBandFloats) : 1.0f);
             const float rest = fabsf(rx - rz);
             rx = ix ? r.stepx : rest;
             rz = ix ? rest : r.stepz;
@@ -187,7 +197,7 @@ __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, in
         const float dt = ix ? rx : rz;
         if (kFast) acc = fmaf(dt, band[li], acc);
         else acc += dt * band[li];
-        accl += dt;                                     // dt * 1.0f
+        accl = length_step(accl, dt, kMask ? band[li + kMaskBandFloats] : 1.0f);
         const float rest = (ix ? rz : rx) - dt;         // the axis not stepped loses dt
         rx = ix ? r.stepx : rest;
         rz = ix ? rest : r.stepz;
@@ -280,16 +290,20 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // (1 ulp; OpenCL allows 2.5 ulp for a division), and the ray sums use fused
 // multiply-adds.  Without kFast the kernel performs the generic kernels'
 // float operations one for one.
+// kMask applies the prior-knowledge mask (ImageHandler.hl:85-89): the ray
+// length sums dt * weight, the update is weight * sum.  A member without a
+// mask gets weight 1 (every update then equals the unmasked kernel's).
 // With at most two rays per thread the kernel fits 128 VGPRs: four waves per
 // SIMD, two 512-thread workgroups per CU (as LDS also allows).
-template <typename S, bool kVec, bool kCap, bool kFast, int RPT>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, int RPT>
 __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
 {
     using IO = GroupIo<S, kVec>;
     constexpr int G = IO::G;
-    constexpr int kGroups = kMaxBandFloats / (G * kFusedThreads);   // band groups per thread (upper bound)
+    constexpr int kBandCap = kMask ? kMaskBandFloats : kMaxBandFloats;
+    constexpr int kGroups = kBandCap / (G * kFusedThreads);   // band groups per thread (upper bound)
     // a fixed-size array puts the band at LDS address 0 (the host sizes every
     // tile to fit it; the VGPR budget already limits a CU to two workgroups)
     __shared__ __attribute__((aligned(16))) float lds[kLdsFloats];
@@ -304,10 +318,14 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     float* band = lds;                                  // band x width
     // per plane row_len column records, 16-byte aligned after the band, then
     // per plane (m4 py, ty) of the current back projection
-    float4* recs = reinterpret_cast<float4*>(lds + ((fv.band * width + 3) & ~3));
+    float* mband = lds + kMaskBandFloats;               // kMask: the band's mask weights
+    float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskBandFloats : 0) + ((fv.band * width + 3) & ~3));
     constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
     float2* pterm = reinterpret_cast<float2*>(recs + np * kRecStride * row_len);
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
+    const ETN_GLOBAL uint8_t* mask = kMask ? global_ptr(fv.mask) : nullptr;
+    // MaskVolume weight of a byte, as the generic kernels compute it
+    auto weight = [](uint32_t m) { return (float)m / 255.0f; };
     const size_t plane_stride = (size_t)nx * fv.ny;
     const size_t plane_base = (size_t)j * nx;
     const int gpr = width / G;                          // groups per band row
@@ -436,6 +454,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     // processed; the scalar path (nx not a multiple of 4) loads at commit time
     constexpr int kPre = kVec ? kGroups : 1;
     typename IO::V pre[kPre];
+    uint32_t mpre[kMask ? kPre : 1];                    // the groups' 4 mask bytes
     auto prefetch = [&](int z0, int z1) {
         if (!kVec) return;
         const int ng = (z1 - z0) * gpr;
@@ -444,7 +463,9 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
             const int gi = k * kFusedThreads + tid;
             if (gi < ng) {
                 const int zz = gi / gpr, xg = gi - zz * gpr;
-                pre[k] = IO::load(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G);
+                const size_t at = (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G;
+                pre[k] = IO::load(vol, at);
+                if (kMask) mpre[k] = mask ? *reinterpret_cast<const ETN_GLOBAL uint32_t*>(mask + at) : 0xffffffffu;
             }
         }
     };
@@ -478,15 +499,25 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     const int zz = gi / gpr, xg = gi - zz * gpr;
                     // one 16-byte LDS store (width and xg * G are multiples of 4)
                     *reinterpret_cast<float4*>(band + zz * width + xg * G) = make_float4(v[0], v[1], v[2], v[3]);
+                    if (kMask) {
+                        const uint32_t m = mpre[k];
+                        *reinterpret_cast<float4*>(mband + zz * width + xg * G) =
+                            make_float4(weight(m & 255u), weight((m >> 8) & 255u), weight((m >> 16) & 255u),
+                                        weight(m >> 24));
+                    }
                 }
             }
         } else {
             for (int gi = tid; gi < ng; gi += kFusedThreads) {
                 const int zz = gi / gpr, xg = gi - zz * gpr;
+                const size_t at = (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G;
                 float v[G];
-                IO::unpack(IO::load(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G), v);
+                IO::unpack(IO::load(vol, at), v);
 #pragma unroll
-                for (int i = 0; i < G; ++i) band[zz * width + xg * G + i] = v[i];
+                for (int i = 0; i < G; ++i) {
+                    band[zz * width + xg * G + i] = v[i];
+                    if (kMask) mband[zz * width + xg * G + i] = mask ? weight(mask[at + i]) : 1.0f;
+                }
             }
         }
         // the prefetch registers are free again: start fetching the next band
@@ -521,7 +552,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 return m8 * pz;
             };
             // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
-            auto update = [&](const float4* pr, float ty, float xt, float zt, float value) {
+            auto update = [&](const float4* pr, float ty, float xt, float zt, float value, float mv) {
                 if (kFast) {
                     // u in the reference's order (u is a few hundred pixels, so a
                     // reordered sum would move the weights by ~1e-4); fract without
@@ -538,7 +569,8 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     const float4 c = pr[i];
                     const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
                     const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
-                    value = (L > kEpsilon) ? value + contrib : value;
+                    const float next = kMask ? fmaf(mv, contrib, value) : value + contrib;
+                    value = (L > kEpsilon) ? next : value;
                     return S::round(value);
                 }
                 float u = xt + zt;
@@ -554,7 +586,8 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 const float wgt = chord / L;
                 const float contrib = Rv * lam * wgt;
                 const float sum = 0.0f + contrib;             // sum over the one sample
-                value = (L > kEpsilon) ? value + sum : value; // += 1 * sum / 1 when used
+                const float add = kMask ? mv * sum : sum;     // weight * sum / 1
+                value = (L > kEpsilon) ? value + add : value; // used == 1 exactly when L > eps
                 return S::round(value);
             };
             const int nrow = z1 - z0;
@@ -579,15 +612,17 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     if constexpr (G == 4) {
                         // four voxels, 16-byte aligned (width and xw are multiples of 4)
                         const float4 in = *reinterpret_cast<const float4*>(cell);
-                        out[0] = update(pr, pt.y, xt[0], zt, in.x);
-                        out[1] = update(pr, pt.y, xt[1], zt, in.y);
-                        out[2] = update(pr, pt.y, xt[2], zt, in.z);
-                        out[3] = update(pr, pt.y, xt[3], zt, in.w);
+                        const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskBandFloats)
+                                                : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+                        out[0] = update(pr, pt.y, xt[0], zt, in.x, mk.x);
+                        out[1] = update(pr, pt.y, xt[1], zt, in.y, mk.y);
+                        out[2] = update(pr, pt.y, xt[2], zt, in.z, mk.z);
+                        out[3] = update(pr, pt.y, xt[3], zt, in.w, mk.w);
                         *reinterpret_cast<float4*>(cell) = make_float4(out[0], out[1], out[2], out[3]);
                     } else {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            out[i] = update(pr, pt.y, xt[i], zt, cell[i]);
+                            out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskBandFloats] : 1.0f);
                             cell[i] = out[i];
                         }
                     }
@@ -609,8 +644,9 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     float out[G];
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
-                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[zz * width + xw + i]);
-                        band[zz * width + xw + i] = out[i];
+                        const int c = zz * width + xw + i;
+                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskBandFloats] : 1.0f);
+                        band[c] = out[i];
                     }
                     if (q == nbp - 1)
                         IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
@@ -625,7 +661,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 const int id = k * kFusedThreads + tid;
-                if (id < nrays) plane_advance<kCap, kFast>(ray[k], band, width, nx, nz, z0, z1);
+                if (id < nrays) plane_advance<kCap, kFast, kMask>(ray[k], band, width, nx, nz, z0, z1);
             }
             if (!kCap && bi + 1 < nbands) {
                 // re-base the LDS offsets on the next band's first row
@@ -691,42 +727,49 @@ struct FusedLaunch {
     size_t lds;
 };
 
-template <typename S, bool kVec, bool kCap, bool kFast>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask>
 void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 {
     if (l.lds > sizeof(float) * (size_t)kLdsFloats) fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kFusedThreads);
     switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, kFast, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, kFast, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, kFast, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, kFast, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
     }
 }
 
-template <typename S, bool kFast>
+template <typename S, bool kFast, bool kMask>
 void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
-    if (vec && !cap) launch_fused_rpt<S, true, false, kFast>(l, stream);
-    else if (vec) launch_fused_rpt<S, true, true, kFast>(l, stream);
-    else if (!cap) launch_fused_rpt<S, false, false, kFast>(l, stream);
-    else launch_fused_rpt<S, false, true, kFast>(l, stream);
+    if (vec && !cap) launch_fused_rpt<S, true, false, kFast, kMask>(l, stream);
+    else if (vec) launch_fused_rpt<S, true, true, kFast, kMask>(l, stream);
+    else if (!cap) launch_fused_rpt<S, false, false, kFast, kMask>(l, stream);
+    else launch_fused_rpt<S, false, true, kFast, kMask>(l, stream);
 }
 
 }  // namespace
 
 // vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049;
-// exact: the generic kernels' operations and rounding order
-void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, const FusedLaunch& l, hipStream_t stream)
+// exact: the generic kernels' operations and rounding order; mask: the launch
+// carries prior-knowledge masks (fast arithmetic only)
+void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask, const FusedLaunch& l,
+                  hipStream_t stream)
 {
     if (exact) {
-        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false>(vec, cap, l, stream);
-        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false>(vec, cap, l, stream);
-        else launch_fused_typed<BF16, false>(vec, cap, l, stream);
+        if (mask) unsupported("fused plane kernel: a mask needs fast arithmetic");
+        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, false>(vec, cap, l, stream);
+        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, false>(vec, cap, l, stream);
+        else launch_fused_typed<BF16, false, false>(vec, cap, l, stream);
+    } else if (mask) {
+        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true, true>(vec, cap, l, stream);
+        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true, true>(vec, cap, l, stream);
+        else launch_fused_typed<BF16, true, true>(vec, cap, l, stream);
     } else {
-        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true>(vec, cap, l, stream);
-        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true>(vec, cap, l, stream);
-        else launch_fused_typed<BF16, true>(vec, cap, l, stream);
+        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true, false>(vec, cap, l, stream);
+        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true, false>(vec, cap, l, stream);
+        else launch_fused_typed<BF16, true, false>(vec, cap, l, stream);
     }
     hip_check(hipGetLastError(), "fused plane kernel launch");
 }
@@ -754,6 +797,7 @@ struct MemberPlan {
     int rows_per_plane = 0;
     int row_len = 0, pad_l = 0;        // staged detector columns, zero columns left of column 0
     int row_floats = 0;                // LDS floats of one plane's staged rows
+    const uint8_t* mask = nullptr;     // prior-knowledge mask (device), or null
     int ndead = 0;
     uint32_t bmax = 0;                 // largest block
     DeviceBuffer rows;                 // row_lo[ny] | row_hi[ny] | dead rows
@@ -775,15 +819,16 @@ int max_band(const MemberPlan& m, int planes)
 {
     if (planes < 1 || planes > dev::kMaxPlanes || (planes & (planes - 1)) || m.ny % planes) return 0;
     if (planes * m.rows_per_plane * m.nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return 0;
-    int band = std::min(m.nz, (dev::kLdsFloats - 3 - planes * (m.row_floats + 2)) / (planes * m.nx));
-    band = std::min(band, dev::kMaxBandFloats / (planes * m.nx));
+    const int masked = m.mask ? dev::kMaskBandFloats : 0;
+    int band = std::min(m.nz, (dev::kLdsFloats - masked - 3 - planes * (m.row_floats + 2)) / (planes * m.nx));
+    band = std::min(band, (m.mask ? dev::kMaskBandFloats : dev::kMaxBandFloats) / (planes * m.nx));
     return std::max(band, 0);
 }
 
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
-           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats) <= dev::kLdsFloats;
+           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr) <= dev::kLdsFloats;
 }
 
 // Without timing: the fewest planes whose rays fill a workgroup, the tallest band.
@@ -833,7 +878,8 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     const VolumeProperties whole = whole_volume(op.getReconstructedVolume()->properties());
     const VolumeProperties& vp = whole;
     if ((d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1) return false;
-    if (d.mask) return false;
+    if (d.mask && exact) return false;   // the exact kernels carry no mask: generic kernels
+    if (reinterpret_cast<uintptr_t>(d.mask) % 4) return false;   // masks are read 4 bytes at a time
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
     const int nu = (int)d.base_geometry.resolution[0], nv = (int)d.base_geometry.resolution[1];
 
@@ -937,7 +983,9 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
         out.row_len = out.pad_l + std::max(nu + dev::kRowPad, bu_hi + 1);
     }
     out.row_floats = out.row_len * (exact ? 8 : 4);
-    if (dev::fused_lds_floats(1, 1, nx, out.row_floats) > dev::kLdsFloats) return false;   // a z row and the rows fit
+    out.mask = d.mask;
+    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr) > dev::kLdsFloats)
+        return false;   // a z row and the rows fit
     if (max_band(out, 1) < 1) return false;   // not even one z row of one plane fits
     out.ndead = (int)dead.size();
     std::vector<int> table(lo);
@@ -1034,7 +1082,7 @@ void build_passes(MemberPlan& mb)
 
 class FusedEngineImpl final : public FusedPlaneEngine {
 public:
-    FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
+    FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask,
                     hipStream_t stream);
     void enqueueIteration(hipStream_t stream) override;
     void check(hipStream_t stream) override;
@@ -1064,14 +1112,15 @@ private:
     std::vector<MemberPlan> members_;
     etn_voxel_type vt_;
     bool vec_, cap_, exact_;
+    bool mask_;                         // every member has a mask (the launches use the mask kernels)
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
     Schedule sched_;
 };
 
 FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
-                                 hipStream_t stream)
-    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact)
+                                 bool mask, hipStream_t stream)
+    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact), mask_(mask)
 {
     diag_.allocate(sizeof(int));
     ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
@@ -1108,6 +1157,7 @@ void FusedEngineImpl::upload(hipStream_t stream)
         fv.row_hi = m.rows.as<int>() + m.ny;
         fv.dead_rows = m.rows.as<int>() + 2 * m.ny;
         fv.ndead = m.ndead;
+        fv.mask = m.mask;
         fv.diag = diag_.as<int>();
         vols.push_back(fv);
     }
@@ -1186,7 +1236,7 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
             for (int j = 0; j < m.ny; j += m.tile.planes) work.push_back({(int)i, j});
             l.rpt = std::max(l.rpt, select_rpt(p.nfp * m.tile.planes * m.rows_per_plane * m.nu));
             l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx,
-                                                                                   m.row_floats));
+                                                                                   m.row_floats, mask_));
             l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
         }
         l.groups = (int)(work.size() - l.work);
@@ -1213,7 +1263,7 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         const dev::FusedLaunch fl{l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
                                   tables + k * members_.size(), l.lds};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
-        dev::launch_fused(vt_, vec_, cap_, exact_, fl, stream);
+        dev::launch_fused(vt_, vec_, cap_, exact_, mask_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
     }
 }
@@ -1343,23 +1393,25 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
     taken->assign(members.size(), false);
     std::vector<MemberPlan> plans;
     etn_voxel_type vt = ETN_VOXEL_FLOAT32;
-    bool vec = true, cap = false;
+    bool vec = true, cap = false, mask = false;
     for (size_t i = 0; i < members.size(); ++i) {
         const etn_reconstruction_desc& d = *members[i].desc;
-        if (!plans.empty() && (d.voxel_type != vt || (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
+        if (!plans.empty() && (d.voxel_type != vt || (d.mask != nullptr) != mask ||
+                               (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
             continue;
         MemberPlan m;
         if (!plan_member(*members[i].op, d, exact, m)) continue;
         if (plans.empty()) {
             vt = d.voxel_type;
             vec = m.nx % 4 == 0;
+            mask = m.mask != nullptr;
         }
         cap = cap || m.nx + m.nz > 2049;
         plans.push_back(std::move(m));
         (*taken)[i] = true;
     }
     if (plans.empty()) return nullptr;
-    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, exact, stream);
+    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, exact, mask, stream);
 }
 
 }  // namespace etn

@@ -148,3 +148,31 @@ def test_autotuned_equals_default_tile(members):
     np.testing.assert_array_equal(outs[0], outs[1])
     _, ref = alone(m, 2)
     np.testing.assert_array_equal(outs[0], ref)
+
+
+def test_batch_with_masks_equals_each_member_alone(members):
+    """Masked members share their own fused engine (the mask kernels); members
+    without a mask keep theirs.  Each volume equals the member run alone on the
+    fused path, and every tile of a masked member gives the same volume."""
+    def mask_for(m, seed):
+        nx, ny, nz = m["volume"].resolution
+        rng = np.random.default_rng(seed)
+        return torch.from_numpy(rng.integers(0, 256, (nz, ny, nx), dtype=np.uint8)).cuda()
+
+    sel = [dict(members[0], mask=mask_for(members[0], 1)), members[1],
+           dict(members[3], mask=mask_for(members[3], 2)), members[2]]
+    b = A.BatchReconstruction(sel)
+    assert all(b.uses_fused_path(i) for i in range(len(sel)))
+    b.run_iterations(2)
+    b.synchronize()
+    for i, m in enumerate(sel):
+        _, ref = alone(m, 2, flags=0)
+        np.testing.assert_array_equal(b.get_reconstructed_volume(i, dtype=torch.float32).cpu().numpy(), ref,
+                                      err_msg="member %d" % i)
+    _, ref = alone(sel[0], 2, flags=0)
+    for tile in [(1, 1), (2, 5), (8, 1)]:
+        r = A.BlockIterativeReconstructionOperator(**sel[0])
+        r.set_tile(*tile)
+        r.run_iterations(2)
+        r.synchronize()
+        np.testing.assert_array_equal(r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy(), ref)
