@@ -130,7 +130,8 @@ def main():
     for _ in range(args.warmup):
         r.run_single_iteration()
     r.synchronize()
-    log("warmup done (fused path: %s)" % fused)
+    shape = (r.tile() + (r.workgroup_threads(),)) if fused else None
+    log("warmup done (fused path: %s, tile (planes, band, threads): %s)" % (fused, shape))
 
     # timed region: K iterations, synchronised on both sides
     torch.cuda.synchronize()
@@ -146,7 +147,9 @@ def main():
     log("residual norm after %d iterations: %.6e" % (args.warmup + args.steps, norm))
 
     block = wl["block_size"] or (1 if wl["algorithm"] == "sart" else N)
-    roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block, masked=mask is not None)
+    # the committed PMC traffic summaries cover the unmasked 512^3 metric workload only
+    roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block, masked=mask is not None,
+                               pmc=args.workload == "metric" and mask is None)
     del r
     torch.cuda.empty_cache()
 
@@ -322,7 +325,8 @@ def run_batch(args):
         b.run_iterations(1)
     b.synchronize()
     tiles = sorted({(comp[i][0],) + b.tile(i) for i in range(len(b))}) if fused else []
-    log("warmup done (fused: %s, tiles (edge, planes, band): %s)" % (fused, tiles))
+    log("warmup done (fused: %s, tiles (edge, planes, band): %s, workgroup threads %s)"
+        % (fused, tiles, b.workgroup_threads(0) if fused else None))
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     b.run_iterations(args.steps)
@@ -434,7 +438,7 @@ def run_learned(args):
     print(json.dumps(out), flush=True)
 
 
-def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False):
+def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True):
     """Average duration of the dominant kernel, measured with HIP events around
     every launch of one extra eager iteration on the reconstruction's stream,
     and its algorithmic bytes per launch.
@@ -465,7 +469,7 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False):
     achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
     log("dominant kernel %s: %.4f ms average over %d launches -> %.1f GB/s algorithmic"
         % (name, dur_ms, launches, achieved))
-    traffic, source, l2 = measured_traffic(kernel, voxel_type) if not masked else (None, None, None)
+    traffic, source, l2 = measured_traffic(kernel, voxel_type) if pmc else (None, None, None)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
             "l2_hit_rate": l2, "kernel": name,

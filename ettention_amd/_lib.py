@@ -132,6 +132,8 @@ SIGNATURES = [
                                                C.POINTER(C.c_uint32)]),
     ("etn_reconstruction_set_tile", _i32, [_vp, C.c_int, C.c_int]),
     ("etn_reconstruction_tile", _i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("etn_reconstruction_set_workgroup_threads", _i32, [_vp, C.c_int]),
+    ("etn_reconstruction_workgroup_threads", _i32, [_vp, C.POINTER(C.c_int)]),
     ("etn_batch_create", _i32, [C.POINTER(ReconstructionDesc), _u32, _u32, _vp, C.POINTER(_vp)]),
     ("etn_batch_destroy", _i32, [_vp]),
     ("etn_batch_run_iterations", _i32, [_vp, _u32]),
@@ -140,6 +142,7 @@ SIGNATURES = [
     ("etn_batch_residual_norm", _i32, [_vp, _u32, C.POINTER(C.c_double)]),
     ("etn_batch_uses_fused_path", _i32, [_vp, _u32, C.POINTER(C.c_int)]),
     ("etn_batch_tile", _i32, [_vp, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("etn_batch_workgroup_threads", _i32, [_vp, _u32, C.POINTER(C.c_int)]),
     ("etn_batch_time_iteration", _i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     ("etn_unet_create", _i32, [_u32, _u32, _u32, _vp, C.POINTER(_vp)]),
     ("etn_unet_destroy", _i32, [_vp]),

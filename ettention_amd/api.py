@@ -268,6 +268,16 @@ class BlockIterativeReconstructionOperator:
         call("etn_reconstruction_tile", self._h, C.byref(p), C.byref(b))
         return p.value, b.value
 
+    def set_workgroup_threads(self, threads):
+        """Fused kernels' workgroup shape: 512 or 256 threads."""
+        call("etn_reconstruction_set_workgroup_threads", self._h, int(threads))
+
+    def workgroup_threads(self):
+        """Threads per fused workgroup in use; runs the autotuner first if it has not run."""
+        t = C.c_int()
+        call("etn_reconstruction_workgroup_threads", self._h, C.byref(t))
+        return t.value
+
     def volume_pointer(self):
         p = C.c_void_p()
         n = C.c_size_t()
@@ -363,6 +373,11 @@ class BatchReconstruction:
         p, b = C.c_int(), C.c_int()
         call("etn_batch_tile", self._h, int(i), C.byref(p), C.byref(b))
         return p.value, b.value
+
+    def workgroup_threads(self, i):
+        t = C.c_int()
+        call("etn_batch_workgroup_threads", self._h, int(i), C.byref(t))
+        return t.value
 
     def time_iteration(self):
         """(total ms of the fused launches, launch count) over one more eager iteration."""

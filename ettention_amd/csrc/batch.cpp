@@ -88,6 +88,12 @@ void BatchReconstruction::tile(uint32_t i, int* planes, int* band) const
     engines_[slots_[i].engine]->tile(slots_[i].index, planes, band);
 }
 
+int BatchReconstruction::workgroupThreads(uint32_t i) const
+{
+    if (!usesFusedPath(i)) unsupported("this batch member does not use the fused plane kernels");
+    return engines_[slots_[i].engine]->threads();
+}
+
 void BatchReconstruction::ensureTuned()
 {
     if (tuned_) return;

@@ -22,6 +22,7 @@ public:
     BlockIterativeReconstructionOperator& member(uint32_t i);
     bool usesFusedPath(uint32_t i) const;
     void tile(uint32_t i, int* planes, int* band) const;
+    int workgroupThreads(uint32_t i) const;   // workgroup shape of the member's fused launches
     // One eager iteration: total milliseconds of the fused launches and their count.
     double timeIteration(uint32_t* launches);
 

@@ -157,6 +157,20 @@ void BlockIterativeReconstructionOperator::tile(int* planes, int* band) const
     fused_->tile(0, planes, band);
 }
 
+void BlockIterativeReconstructionOperator::setWorkgroupThreads(int threads)
+{
+    if (!fused_) unsupported("this reconstruction does not use the fused plane kernels");
+    invalidate_graph();
+    fused_->setThreads(threads, stream_);
+    tuned_ = true;
+}
+
+int BlockIterativeReconstructionOperator::workgroupThreads() const
+{
+    if (!fused_) unsupported("this reconstruction does not use the fused plane kernels");
+    return fused_->threads();
+}
+
 void BlockIterativeReconstructionOperator::runNiterations(uint32_t iterations)
 {
     if (iterations == 0) return;

@@ -49,6 +49,8 @@ public:
     void ensureTuned() override;
     void setTile(int planes, int band) override;
     void tile(int* planes, int* band) const override;
+    void setWorkgroupThreads(int threads) override;
+    int workgroupThreads() const override;
     // The launches of one iteration on stream(), without graph capture.
     void enqueueIteration();
     // Called on the host after each block's back projection has been enqueued

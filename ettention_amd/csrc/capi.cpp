@@ -401,6 +401,24 @@ etn_status etn_reconstruction_tile(etn_reconstruction* r, int* planes, int* band
     });
 }
 
+etn_status etn_reconstruction_set_workgroup_threads(etn_reconstruction* r, int threads)
+{
+    return guarded([&] {
+        need(r, "reconstruction");
+        r->op->setWorkgroupThreads(threads);
+    });
+}
+
+etn_status etn_reconstruction_workgroup_threads(etn_reconstruction* r, int* threads)
+{
+    return guarded([&] {
+        need(r, "reconstruction");
+        need(threads, "threads");
+        r->op->ensureTuned();
+        *threads = r->op->workgroupThreads();
+    });
+}
+
 etn_status etn_batch_create(const etn_reconstruction_desc* members, uint32_t count, uint32_t flags, void* stream,
                             etn_batch** out)
 {
@@ -470,6 +488,15 @@ etn_status etn_batch_tile(etn_batch* b, uint32_t member, int* planes, int* band)
         need(planes, "planes");
         need(band, "band");
         b->batch->tile(member, planes, band);
+    });
+}
+
+etn_status etn_batch_workgroup_threads(etn_batch* b, uint32_t member, int* threads)
+{
+    return guarded([&] {
+        need(b, "batch");
+        need(threads, "threads");
+        *threads = b->batch->workgroupThreads(member);
     });
 }
 

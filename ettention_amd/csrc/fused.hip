@@ -34,17 +34,37 @@
 namespace etn {
 namespace dev {
 
-constexpr int kFusedThreads = 512;
 constexpr int kMaxFusedFp = 16;      // forward projections per pass
 constexpr int kMaxFusedBp = 8;       // back projections per pass
 constexpr int kMaxRaysPerThread = 8;
 constexpr int kMaxPlanes = 8;        // planes per workgroup
-constexpr int kLdsFloats = 16384;    // 64 KiB per workgroup
 constexpr int kRowPad = 2;           // zero taps on each side of a staged detector row
-constexpr int kMaxBandFloats = 12288;   // band voxels: the vector path prefetches 6 groups of 4 per thread
-// With a prior-knowledge mask the band holds half as many voxels and the
-// voxels' mask weights follow at this fixed LDS offset (floats).
-constexpr int kMaskBandFloats = kMaxBandFloats / 2;
+
+// Workgroup shapes.  512 threads with a 64 KiB LDS array run two workgroups
+// per CU; 256 threads with 40 KiB run four, whose independent barrier phases
+// overlap better when a pass carries many projections, at half the band.
+// kBand: band voxels (the vector path prefetches 6 groups of 4 per thread).
+// With a prior-knowledge mask the band holds kBand / 2 voxels and the voxels'
+// mask weights follow at LDS offset kBand / 2 (floats).
+template <int kThreads>
+struct FusedShape;
+template <>
+struct FusedShape<512> {
+    static constexpr int kLds = 16384;
+    static constexpr int kBand = 12288;
+};
+template <>
+struct FusedShape<256> {
+    static constexpr int kLds = 10240;
+    static constexpr int kBand = 6144;
+};
+constexpr int kMaxThreads = 512;
+__host__ __device__ inline int shape_lds_floats(int threads) { return threads == 256 ? FusedShape<256>::kLds : FusedShape<512>::kLds; }
+__host__ __device__ inline int shape_band_floats(int threads, bool mask)
+{
+    const int b = threads == 256 ? FusedShape<256>::kBand : FusedShape<512>::kBand;
+    return mask ? b / 2 : b;
+}
 
 struct FusedFp {
     float src[3], h[3], v[3], ray[3];
@@ -94,13 +114,14 @@ struct FusedWork {
     int plane;                   // first plane of the workgroup's group
 };
 
-// LDS floats a workgroup uses: the band (with a mask: the band, at
-// kMaskBandFloats the band's mask weights), then (16-byte aligned) the staged
-// detector records of each plane (row_floats per plane: 8 floats per padded
-// column with exact arithmetic, 4 with fast) and a (m4 py, ty) pair per plane.
-__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats, bool mask)
+// LDS floats a workgroup of `threads` uses: the band (with a mask: the band,
+// at offset shape_band_floats(threads, true) the band's mask weights), then
+// (16-byte aligned) the staged detector records of each plane (row_floats per
+// plane: 8 floats per padded column with exact arithmetic, 4 with fast) and a
+// (m4 py, ty) pair per plane.
+__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats, bool mask, int threads)
 {
-    return (mask ? kMaskBandFloats : 0) + ((band * planes * nx + 3) & ~3) + planes * (row_floats + 2);
+    return (mask ? shape_band_floats(threads, true) : 0) + ((band * planes * nx + 3) & ~3) + planes * (row_floats + 2);
 }
 
 namespace {
@@ -132,8 +153,8 @@ struct PlaneRay {
 // one, so a ray can take at most nx + nz - 1 steps inside the volume.
 // kFast accumulates dt * value with a fused multiply-add (one rounding instead
 // of two, as an OpenCL compiler may contract it).  kMask weights the ray
-// length by the voxel's mask weight, stored kMaskBandFloats after the voxel.
-template <bool kCap, bool kFast, bool kMask>
+// length by the voxel's mask weight, stored kMaskOff floats after the voxel.
+template <bool kCap, bool kFast, bool kMask, int kMaskOff>
 __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, int width, int nx, int nz, int zlo,
                                               int zhi)
 {
@@ -158,7 +179,7 @@ __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, in
             const float value = *reinterpret_cast<const float*>(bytes + li);
             if (kFast) acc = fmaf(dt, value, acc);
             else acc += dt * value;
-            accl = length_step(accl, dt, kMask ? *reinterpret_cast<const float*>(bytes + li + 4 * kMaskBandFloats) : 1.0f);
+            accl = length_step(accl, dt, kMask ? *reinterpret_cast<const float*>(bytes + li + 4 * kMaskOff) : 1.0f);
             const float rest = fabsf(rx - rz);
             rx = ix ? r.stepx : rest;
             rz = ix ? rest : r.stepz;
@@ -197,7 +218,7 @@ __device__ __forceinline__ void plane_advance(PlaneRay& r, const float* band, in
         const float dt = ix ? rx : rz;
         if (kFast) acc = fmaf(dt, band[li], acc);
         else acc += dt * band[li];
-        accl = length_step(accl, dt, kMask ? band[li + kMaskBandFloats] : 1.0f);
+        accl = length_step(accl, dt, kMask ? band[li + kMaskOff] : 1.0f);
         const float rest = (ix ? rz : rx) - dt;         // the axis not stepped loses dt
         rx = ix ? r.stepx : rest;
         rz = ix ? rest : r.stepz;
@@ -295,18 +316,20 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // mask gets weight 1 (every update then equals the unmasked kernel's).
 // With at most two rays per thread the kernel fits 128 VGPRs: four waves per
 // SIMD, two 512-thread workgroups per CU (as LDS also allows).
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, int RPT>
-__global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, int kThreads, int RPT>
+__global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
 {
     using IO = GroupIo<S, kVec>;
+    using Shape = FusedShape<kThreads>;
     constexpr int G = IO::G;
-    constexpr int kBandCap = kMask ? kMaskBandFloats : kMaxBandFloats;
-    constexpr int kGroups = kBandCap / (G * kFusedThreads);   // band groups per thread (upper bound)
+    constexpr int kMaskOff = Shape::kBand / 2;          // LDS floats from a voxel to its mask weight
+    constexpr int kBandCap = kMask ? kMaskOff : Shape::kBand;
+    constexpr int kGroups = kBandCap / (G * kThreads);  // band groups per thread (upper bound)
     // a fixed-size array puts the band at LDS address 0 (the host sizes every
-    // tile to fit it; the VGPR budget already limits a CU to two workgroups)
-    __shared__ __attribute__((aligned(16))) float lds[kLdsFloats];
+    // tile to fit it; the VGPR budget limits the workgroups per CU as much)
+    __shared__ __attribute__((aligned(16))) float lds[Shape::kLds];
     const FusedWork item = work[blockIdx.x];
     const ETN_GLOBAL FusedVolume& fv = global_ptr(vols)[item.member];
     const ETN_GLOBAL FusedPass* pass = global_ptr(global_ptr(passes)[item.member]);
@@ -318,8 +341,8 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     float* band = lds;                                  // band x width
     // per plane row_len column records, 16-byte aligned after the band, then
     // per plane (m4 py, ty) of the current back projection
-    float* mband = lds + kMaskBandFloats;               // kMask: the band's mask weights
-    float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskBandFloats : 0) + ((fv.band * width + 3) & ~3));
+    float* mband = lds + kMaskOff;                      // kMask: the band's mask weights
+    float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskOff : 0) + ((fv.band * width + 3) & ~3));
     constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
     float2* pterm = reinterpret_cast<float2*>(recs + np * kRecStride * row_len);
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
@@ -354,7 +377,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     PlaneRay ray[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-        const int id = k * kFusedThreads + tid;
+        const int id = k * kThreads + tid;
         ray[k].live = 0;
         ray[k].acc = 0.0f;
         ray[k].accl = 0.0f;
@@ -410,7 +433,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
         const ETN_GLOBAL FusedBp& bp = pass->bp[q];
         const ETN_GLOBAL float* res = global_ptr(bp.residual);
         const ETN_GLOBAL float* len = global_ptr(bp.length);
-        for (int e = tid; e < np * row_len; e += kFusedThreads) {
+        for (int e = tid; e < np * row_len; e += kThreads) {
             const int p = e / row_len, k = e - p * row_len, u = k - pad_l;
             float py, ty;
             const int row = plane_v(bp, p, py, ty);
@@ -460,7 +483,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
         const int ng = (z1 - z0) * gpr;
 #pragma unroll
         for (int k = 0; k < kPre; ++k) {
-            const int gi = k * kFusedThreads + tid;
+            const int gi = k * kThreads + tid;
             if (gi < ng) {
                 const int zz = gi / gpr, xg = gi - zz * gpr;
                 const size_t at = (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G;
@@ -492,7 +515,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
         if (kVec) {
 #pragma unroll
             for (int k = 0; k < kPre; ++k) {
-                const int gi = k * kFusedThreads + tid;
+                const int gi = k * kThreads + tid;
                 if (gi < ng) {
                     float v[G];
                     IO::unpack(pre[k], v);
@@ -508,7 +531,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 }
             }
         } else {
-            for (int gi = tid; gi < ng; gi += kFusedThreads) {
+            for (int gi = tid; gi < ng; gi += kThreads) {
                 const int zz = gi / gpr, xg = gi - zz * gpr;
                 const size_t at = (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G;
                 float v[G];
@@ -591,9 +614,9 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 return S::round(value);
             };
             const int nrow = z1 - z0;
-            if (kFusedThreads % gpr == 0) {
+            if (kThreads % gpr == 0) {
                 // every thread keeps the same x group (and plane) in every band row
-                const int xw = (tid % gpr) * G, dz = kFusedThreads / gpr;
+                const int xw = (tid % gpr) * G, dz = kThreads / gpr;
                 int p = 0, x0 = xw;
                 if (np > 1) {
                     p = xw / nx;
@@ -612,7 +635,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     if constexpr (G == 4) {
                         // four voxels, 16-byte aligned (width and xw are multiples of 4)
                         const float4 in = *reinterpret_cast<const float4*>(cell);
-                        const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskBandFloats)
+                        const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskOff)
                                                 : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
                         out[0] = update(pr, pt.y, xt[0], zt, in.x, mk.x);
                         out[1] = update(pr, pt.y, xt[1], zt, in.y, mk.y);
@@ -622,7 +645,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                     } else {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskBandFloats] : 1.0f);
+                            out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskOff] : 1.0f);
                             cell[i] = out[i];
                         }
                     }
@@ -631,7 +654,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
                 }
             } else {
 #pragma unroll 1
-                for (int gi = tid; gi < ng; gi += kFusedThreads) {
+                for (int gi = tid; gi < ng; gi += kThreads) {
                     const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
                     int p = 0, x0 = xw;
                     if (np > 1) {
@@ -645,7 +668,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int c = zz * width + xw + i;
-                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskBandFloats] : 1.0f);
+                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskOff] : 1.0f);
                         band[c] = out[i];
                     }
                     if (q == nbp - 1)
@@ -660,8 +683,8 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
         if (nfp > 0) {
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
-                const int id = k * kFusedThreads + tid;
-                if (id < nrays) plane_advance<kCap, kFast, kMask>(ray[k], band, width, nx, nz, z0, z1);
+                const int id = k * kThreads + tid;
+                if (id < nrays) plane_advance<kCap, kFast, kMask, kMaskOff>(ray[k], band, width, nx, nz, z0, z1);
             }
             if (!kCap && bi + 1 < nbands) {
                 // re-base the LDS offsets on the next band's first row
@@ -678,7 +701,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
     // ---- residual (+ long-object compensation) per pixel ----
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-        const int id = k * kFusedThreads + tid;
+        const int id = k * kThreads + tid;
         int q, p, w, u;
         if (id < nrays && decode(id, q, p, w, u)) {
             const ETN_GLOBAL FusedFp& f = pass->fp[q];
@@ -703,7 +726,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
             const int w = global_ptr(fv.dead_rows)[d];
             for (int q = 0; q < nfp; ++q) {
                 const ETN_GLOBAL FusedFp& f = pass->fp[q];
-                for (int u = tid; u < nu; u += kFusedThreads) {
+                for (int u = tid; u < nu; u += kThreads) {
                     float pv = 0.0f + 0.0f / 1.0f;
                     float l = 0.0f + 0.0f / 1.0f;
                     if (f.use_loc) {
@@ -719,6 +742,7 @@ __global__ void __launch_bounds__(kFusedThreads, RPT <= 2 ? 4 : 1)
 }
 
 struct FusedLaunch {
+    int threads;                 // workgroup shape: 512 or 256
     int rpt;
     int groups;
     const FusedVolume* vols;
@@ -727,17 +751,25 @@ struct FusedLaunch {
     size_t lds;
 };
 
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, int kThreads>
+void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
+{
+    if (l.lds > sizeof(float) * (size_t)FusedShape<kThreads>::kLds) fail("fused plane kernel: tile exceeds the LDS array");
+    const dim3 grid(l.groups), block(kThreads);
+    switch (l.rpt) {
+        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+    }
+}
+
 template <typename S, bool kVec, bool kCap, bool kFast, bool kMask>
 void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 {
-    if (l.lds > sizeof(float) * (size_t)kLdsFloats) fail("fused plane kernel: tile exceeds the LDS array");
-    const dim3 grid(l.groups), block(kFusedThreads);
-    switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-    }
+    if (l.threads == 256) launch_fused_shape<S, kVec, kCap, kFast, kMask, 256>(l, stream);
+    else if (l.threads == 512) launch_fused_shape<S, kVec, kCap, kFast, kMask, 512>(l, stream);
+    else fail("fused plane kernel: workgroups have 256 or 512 threads");
 }
 
 template <typename S, bool kFast, bool kMask>
@@ -798,6 +830,7 @@ struct MemberPlan {
     int row_len = 0, pad_l = 0;        // staged detector columns, zero columns left of column 0
     int row_floats = 0;                // LDS floats of one plane's staged rows
     const uint8_t* mask = nullptr;     // prior-knowledge mask (device), or null
+    int threads = dev::kMaxThreads;    // workgroup shape of the engine's launches
     int ndead = 0;
     uint32_t bmax = 0;                 // largest block
     DeviceBuffer rows;                 // row_lo[ny] | row_hi[ny] | dead rows
@@ -807,28 +840,33 @@ struct MemberPlan {
     int offset = 0;                    // launch that carries the member's first pass
 };
 
-int select_rpt(int rays)
+int select_rpt(int rays, int threads)
 {
     int rpt = 1;
-    while (rpt * dev::kFusedThreads < rays && rpt < dev::kMaxRaysPerThread) rpt *= 2;
+    while (rpt * threads < rays && rpt < dev::kMaxRaysPerThread) rpt *= 2;
     return rpt;
 }
 
-// Largest band for `planes` planes per workgroup, 0 when the tile does not fit.
-int max_band(const MemberPlan& m, int planes)
+// Largest band for `planes` planes per workgroup of `threads`, 0 when the tile does not fit.
+int max_band(const MemberPlan& m, int planes, int threads)
 {
     if (planes < 1 || planes > dev::kMaxPlanes || (planes & (planes - 1)) || m.ny % planes) return 0;
-    if (planes * m.rows_per_plane * m.nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return 0;
-    const int masked = m.mask ? dev::kMaskBandFloats : 0;
-    int band = std::min(m.nz, (dev::kLdsFloats - masked - 3 - planes * (m.row_floats + 2)) / (planes * m.nx));
-    band = std::min(band, (m.mask ? dev::kMaskBandFloats : dev::kMaxBandFloats) / (planes * m.nx));
+    if (planes * m.rows_per_plane * m.nu > threads * dev::kMaxRaysPerThread) return 0;
+    const bool mask = m.mask != nullptr;
+    const int masked = mask ? dev::shape_band_floats(threads, true) : 0;
+    int band = std::min(m.nz, (dev::shape_lds_floats(threads) - masked - 3 - planes * (m.row_floats + 2)) /
+                                  (planes * m.nx));
+    band = std::min(band, dev::shape_band_floats(threads, mask) / (planes * m.nx));
     return std::max(band, 0);
 }
+
+int max_band(const MemberPlan& m, int planes) { return max_band(m, planes, m.threads); }
 
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
-           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr) <= dev::kLdsFloats;
+           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr, m.threads) <=
+               dev::shape_lds_floats(m.threads);
 }
 
 // Without timing: the fewest planes whose rays fill a workgroup, the tallest band.
@@ -839,7 +877,7 @@ Tile default_tile(const MemberPlan& m)
         const int b = max_band(m, p);
         if (b < 1) continue;
         best = {p, b};
-        if (p * m.rows_per_plane * m.nu >= dev::kFusedThreads) break;
+        if (p * m.rows_per_plane * m.nu >= m.threads) break;
     }
     return best;
 }
@@ -943,7 +981,7 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     }
     int rows_per_plane = 0;
     for (int jj = 0; jj < ny; ++jj) rows_per_plane = std::max(rows_per_plane, hi[jj] - lo[jj]);
-    if (rows_per_plane < 1 || rows_per_plane * nu > dev::kFusedThreads * dev::kMaxRaysPerThread) return false;
+    if (rows_per_plane < 1 || rows_per_plane * nu > dev::kMaxThreads * dev::kMaxRaysPerThread) return false;
 
     out.op = &op;
     out.nx = nx;
@@ -984,7 +1022,9 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     }
     out.row_floats = out.row_len * (exact ? 8 : 4);
     out.mask = d.mask;
-    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr) > dev::kLdsFloats)
+    out.threads = dev::kMaxThreads;
+    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr, out.threads) >
+        dev::shape_lds_floats(out.threads))
         return false;   // a z row and the rows fit
     if (max_band(out, 1) < 1) return false;   // not even one z row of one plane fits
     out.ndead = (int)dead.size();
@@ -1006,7 +1046,7 @@ void build_passes(MemberPlan& mb)
 {
     BlockIterativeReconstructionOperator& op = *mb.op;
     mb.passes.clear();
-    const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, dev::kFusedThreads * dev::kMaxRaysPerThread /
+    const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, mb.threads * dev::kMaxRaysPerThread /
                                                                   (mb.tile.planes * mb.rows_per_plane * mb.nu)));
     auto image = [&](uint32_t parity, uint32_t slot, int which) {
         return mb.images.as<float>() + mb.pixels * ((((size_t)parity * mb.bmax + slot) * 2) + which);
@@ -1090,6 +1130,8 @@ public:
     void autotune(hipStream_t stream) override;
     void setTile(size_t member, int planes, int band, hipStream_t stream) override;
     void tile(size_t member, int* planes, int* band) const override;
+    void setThreads(int threads, hipStream_t stream) override;
+    int threads() const override { return threads_; }
     size_t memberCount() const override { return members_.size(); }
 
 private:
@@ -1108,11 +1150,16 @@ private:
     void level_offsets();
     void build_schedule(const std::vector<size_t>& ids, Schedule& out, hipStream_t stream) const;
     void enqueue(const Schedule& s, size_t first, size_t last, hipStream_t stream, std::vector<hipEvent_t>* ev) const;
+    bool threadsFit(int threads) const;
+    void useThreads(int threads);
+    void tuneTiles(hipStream_t stream);
+    float timeLaunches(const Schedule& s, hipStream_t stream) const;
 
     std::vector<MemberPlan> members_;
     etn_voxel_type vt_;
     bool vec_, cap_, exact_;
     bool mask_;                         // every member has a mask (the launches use the mask kernels)
+    int threads_ = dev::kMaxThreads;    // workgroup shape of every launch
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
     Schedule sched_;
@@ -1234,9 +1281,9 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
             const dev::FusedPass& p = m.passes[pi];
             tables[k * members_.size() + i] = base + pass_base_[i] + pi;
             for (int j = 0; j < m.ny; j += m.tile.planes) work.push_back({(int)i, j});
-            l.rpt = std::max(l.rpt, select_rpt(p.nfp * m.tile.planes * m.rows_per_plane * m.nu));
+            l.rpt = std::max(l.rpt, select_rpt(p.nfp * m.tile.planes * m.rows_per_plane * m.nu, threads_));
             l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx,
-                                                                                   m.row_floats, mask_));
+                                                                                   m.row_floats, mask_, threads_));
             l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
         }
         l.groups = (int)(work.size() - l.work);
@@ -1260,7 +1307,7 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
     for (size_t k = first; k < last; ++k) {
         const Launch& l = s.launches[k];
         if (l.groups == 0) continue;
-        const dev::FusedLaunch fl{l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
+        const dev::FusedLaunch fl{threads_, l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
                                   tables + k * members_.size(), l.lds};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
         dev::launch_fused(vt_, vec_, cap_, exact_, mask_, fl, stream);
@@ -1312,16 +1359,95 @@ void FusedEngineImpl::tile(size_t member, int* planes, int* band) const
     *band = members_[member].tile.band;
 }
 
-void FusedEngineImpl::autotune(hipStream_t stream)
+// Every member keeps a fused form with `threads` per workgroup: its rays and
+// a z row of one plane fit.
+bool FusedEngineImpl::threadsFit(int threads) const
 {
-    // members of one size share a tile; each size is timed on its own
+    if (threads != 256 && threads != 512) return false;
+    for (const MemberPlan& m : members_)
+        if (max_band(m, 1, threads) < 1) return false;
+    return true;
+}
+
+// Switches every member to `threads`; tiles that no longer fit fall back to the default.
+void FusedEngineImpl::useThreads(int threads)
+{
+    threads_ = threads;
+    for (MemberPlan& m : members_) {
+        m.threads = threads;
+        if (!tile_fits(m, m.tile)) m.tile = default_tile(m);
+    }
+}
+
+void FusedEngineImpl::setThreads(int threads, hipStream_t stream)
+{
+    if (!threadsFit(threads)) fail("workgroup size must be 256 or 512 threads and fit every member");
+    useThreads(threads);
+    upload(stream);
+}
+
+// Event time of launches 1 .. kTimed of a schedule (launch 0 runs untimed); -1
+// when the schedule has fewer than two launches.
+float FusedEngineImpl::timeLaunches(const Schedule& s, hipStream_t stream) const
+{
+    constexpr size_t kTimed = 4;
+    const size_t n = s.launches.size();
+    if (n < 2) return -1.0f;
+    hipEvent_t a, b;
+    ETN_HIP(hipEventCreate(&a));
+    ETN_HIP(hipEventCreate(&b));
+    enqueue(s, 0, 1, stream, nullptr);
+    ETN_HIP(hipEventRecord(a, stream));
+    enqueue(s, 1, std::min(n, 1 + kTimed), stream, nullptr);
+    ETN_HIP(hipEventRecord(b, stream));
+    ETN_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    ETN_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+// Members of one size share a tile; each size is timed on its own, under the
+// current workgroup shape.
+void FusedEngineImpl::tuneTiles(hipStream_t stream)
+{
     std::map<std::tuple<int, int, int, int, int, int>, std::vector<size_t>> classes;
     for (size_t i = 0; i < members_.size(); ++i) {
         const MemberPlan& m = members_[i];
         classes[{m.nx, m.ny, m.nz, m.nu, m.nv, m.rows_per_plane}].push_back(i);
     }
-    bool any = false;
-    for (auto& kv : classes) any = any || candidate_tiles(members_[kv.second[0]]).size() > 1;
+    for (auto& kv : classes) {
+        const std::vector<size_t>& ids = kv.second;
+        const std::vector<Tile> cands = candidate_tiles(members_[ids[0]]);
+        if (cands.size() <= 1) continue;
+        float best_ms = -1.0f;
+        Tile best = members_[ids[0]].tile;
+        for (const Tile& t : cands) {
+            for (size_t i : ids) members_[i].tile = t;
+            upload(stream);
+            for (size_t i : ids) members_[i].offset = 0;
+            Schedule s;
+            build_schedule(ids, s, stream);
+            const float ms = timeLaunches(s, stream);
+            if (ms >= 0.0f && (best_ms < 0.0f || ms < best_ms)) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+        for (size_t i : ids) members_[i].tile = best;
+    }
+}
+
+// For each workgroup shape that fits every member: tune the tiles, then time
+// the engine's whole schedule; keep the fastest shape with its tiles.
+void FusedEngineImpl::autotune(hipStream_t stream)
+{
+    std::vector<int> shapes;
+    for (int t : {512, 256})
+        if (threadsFit(t)) shapes.push_back(t);
+    bool any = shapes.size() > 1;
+    for (const MemberPlan& m : members_) any = any || candidate_tiles(m).size() > 1;
     if (!any) {
         upload(stream);
         return;
@@ -1340,41 +1466,24 @@ void FusedEngineImpl::autotune(hipStream_t stream)
                                hipMemcpyDeviceToDevice, stream));
         at += bytes[i];
     }
-    constexpr size_t kTimed = 4;   // launches timed per candidate, after one untimed
-    hipEvent_t a, b;
-    ETN_HIP(hipEventCreate(&a));
-    ETN_HIP(hipEventCreate(&b));
-    for (auto& kv : classes) {
-        const std::vector<size_t>& ids = kv.second;
-        const std::vector<Tile> cands = candidate_tiles(members_[ids[0]]);
-        if (cands.size() <= 1) continue;
-        double best_ms = 0.0;
-        Tile best = members_[ids[0]].tile;
-        for (const Tile& t : cands) {
-            for (size_t i : ids) members_[i].tile = t;
-            upload(stream);
-            for (size_t i : ids) members_[i].offset = 0;
-            Schedule s;
-            build_schedule(ids, s, stream);
-            const size_t n = s.launches.size();
-            if (n < 2) continue;
-            const size_t last = std::min(n, 1 + kTimed);
-            enqueue(s, 0, 1, stream, nullptr);
-            ETN_HIP(hipEventRecord(a, stream));
-            enqueue(s, 1, last, stream, nullptr);
-            ETN_HIP(hipEventRecord(b, stream));
-            ETN_HIP(hipEventSynchronize(b));
-            float ms = 0.0f;
-            ETN_HIP(hipEventElapsedTime(&ms, a, b));
-            if (best_ms == 0.0 || ms < best_ms) {
-                best_ms = ms;
-                best = t;
-            }
+    int best_threads = threads_;
+    std::vector<Tile> best_tiles;
+    float best_ms = -1.0f;
+    for (int t : shapes) {
+        useThreads(t);
+        for (MemberPlan& m : members_) m.tile = default_tile(m);
+        tuneTiles(stream);
+        upload(stream);
+        const float ms = shapes.size() > 1 ? timeLaunches(sched_, stream) : 0.0f;
+        if (best_tiles.empty() || (ms >= 0.0f && ms < best_ms)) {
+            best_ms = ms;
+            best_threads = t;
+            best_tiles.clear();
+            for (const MemberPlan& m : members_) best_tiles.push_back(m.tile);
         }
-        for (size_t i : ids) members_[i].tile = best;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    useThreads(best_threads);
+    for (size_t i = 0; i < members_.size(); ++i) members_[i].tile = best_tiles[i];
     at = 0;
     for (size_t i = 0; i < members_.size(); ++i) {
         ETN_HIP(hipMemcpyAsync(members_[i].op->getReconstructedVolume()->data(), saved.as<char>() + at, bytes[i],

@@ -44,14 +44,20 @@ public:
     // One eager iteration with an event pair around every launch.
     virtual std::vector<LaunchTime> timeIteration(hipStream_t stream) = 0;
     // Times the candidate tiles (planes per workgroup, z rows per band) of
-    // every member size on the members' own data and keeps the fastest.  The
-    // volumes are restored afterwards; tiles never change results.
+    // every member size on the members' own data, for each workgroup shape,
+    // and keeps the fastest.  The volumes are restored afterwards; tiles never
+    // change results.
     virtual void autotune(hipStream_t stream) = 0;
     // Index is the member's position among the engine's members (in the order
     // given to create).  Throws ETN_ERROR_INVALID_ARGUMENT for a tile that
     // does not fit; reallocates the launch tables (no captured graph may use them).
     virtual void setTile(size_t member, int planes, int band, hipStream_t stream) = 0;
     virtual void tile(size_t member, int* planes, int* band) const = 0;
+    // Workgroup shape of every launch: 512 threads (64 KiB LDS, two workgroups
+    // per CU) or 256 (40 KiB, four per CU).  The autotuner picks one; results
+    // never depend on it.  Tiles that no longer fit revert to the default.
+    virtual void setThreads(int threads, hipStream_t stream) = 0;
+    virtual int threads() const = 0;
     virtual size_t memberCount() const = 0;
 };
 

@@ -30,6 +30,8 @@ public:
     virtual void ensureTuned() {}
     virtual void setTile(int, int) { unsupported("this reconstruction does not use the fused plane kernels"); }
     virtual void tile(int*, int*) const { unsupported("this reconstruction does not use the fused plane kernels"); }
+    virtual void setWorkgroupThreads(int) { unsupported("this reconstruction does not use the fused plane kernels"); }
+    virtual int workgroupThreads() const { unsupported("this reconstruction does not use the fused plane kernels"); }
 };
 
 }  // namespace etn

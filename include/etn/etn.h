@@ -238,9 +238,18 @@ etn_status etn_reconstruction_time_kernels(etn_reconstruction* r, int* kernel, d
  * ETN_RECON_FLAG_NO_AUTOTUNE is set; setting a tile skips that.  Tiles never
  * change results.  ETN_ERROR_UNSUPPORTED on the generic path,
  * ETN_ERROR_INVALID_ARGUMENT for a tile that does not fit (planes must be
- * 1, 2, 4 or 8 and divide ny; band and detector rows must fit 64 KiB of LDS). */
+ * 1, 2, 4 or 8 and divide ny; band and detector rows must fit the LDS array
+ * of the workgroup shape: 64 KiB at 512 threads, 40 KiB at 256). */
 etn_status etn_reconstruction_set_tile(etn_reconstruction* r, int planes, int band);
 etn_status etn_reconstruction_tile(etn_reconstruction* r, int* planes, int* band);
+/* Extension (no reference counterpart): threads per workgroup of the fused
+ * kernels, 512 (two workgroups per CU) or 256 (four per CU, half the band).
+ * The autotuner chooses it with the tile; setting it skips the autotuner, and
+ * a tile that no longer fits reverts to the default tile.  Never changes
+ * results.  ETN_ERROR_UNSUPPORTED on the generic path,
+ * ETN_ERROR_INVALID_ARGUMENT for another size or one the volume does not fit. */
+etn_status etn_reconstruction_set_workgroup_threads(etn_reconstruction* r, int threads);
+etn_status etn_reconstruction_workgroup_threads(etn_reconstruction* r, int* threads);
 
 /* ---------------------------------------------------------------------------
  * Batched reconstructions (extension, no reference counterpart): many
@@ -265,6 +274,8 @@ etn_status etn_batch_volume(etn_batch* b, uint32_t member, void** device_volume,
 etn_status etn_batch_residual_norm(etn_batch* b, uint32_t member, double* out);
 etn_status etn_batch_uses_fused_path(etn_batch* b, uint32_t member, int* out);
 etn_status etn_batch_tile(etn_batch* b, uint32_t member, int* planes, int* band);
+/* Threads per workgroup of the fused launches that carry `member`. */
+etn_status etn_batch_workgroup_threads(etn_batch* b, uint32_t member, int* threads);
 /* One more eager iteration of every member with events around every fused
  * launch: total milliseconds of those launches and their count.  Advances the batch. */
 etn_status etn_batch_time_iteration(etn_batch* b, double* fused_ms, uint32_t* launches);

@@ -176,3 +176,53 @@ def test_batch_with_masks_equals_each_member_alone(members):
         r.run_iterations(2)
         r.synchronize()
         np.testing.assert_array_equal(r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("tile", [(1, 1), (1, 6), (2, 5), (4, 2), (8, 1)])
+def test_256_thread_workgroups_give_the_same_volume(members, tile):
+    # the 256-thread shape (four workgroups per CU, 40 KiB LDS) with exact
+    # arithmetic: bit-identical to the generic kernels for every tile
+    m = members[0]   # 64^3, 41 tilts
+    _, ref = alone(m, 2)
+    r = A.BlockIterativeReconstructionOperator(flags=EXACT, **m)
+    r.set_workgroup_threads(256)
+    r.set_tile(*tile)
+    assert r.workgroup_threads() == 256 and r.tile() == tile
+    r.run_iterations(2)
+    r.synchronize()
+    np.testing.assert_array_equal(r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("which", [1, 2, 4, 5])
+def test_workgroup_shapes_agree_with_fast_arithmetic(members, which):
+    # default arithmetic, block / LOC / odd nx / bf16 members, with and without
+    # a mask: 256 and 512 threads per workgroup give the same volume bit for bit
+    m = members[which]
+    nx, ny, nz = m["volume"].resolution
+    rng = np.random.default_rng(which)
+    for mask in (None, torch.from_numpy(rng.integers(0, 256, (nz, ny, nx), dtype=np.uint8)).cuda()):
+        vols = []
+        for threads in (512, 256):
+            r = A.BlockIterativeReconstructionOperator(mask=mask, **m)
+            r.set_workgroup_threads(threads)
+            r.run_iterations(2)
+            r.synchronize()
+            vols.append(r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy())
+        np.testing.assert_array_equal(vols[0], vols[1])
+
+
+def test_workgroup_threads_are_validated(members):
+    r = A.BlockIterativeReconstructionOperator(**members[3])
+    for threads in (0, 128, 384, 1024):
+        with pytest.raises(E.InvalidArgument):
+            r.set_workgroup_threads(threads)
+    r.set_workgroup_threads(256)
+    assert r.workgroup_threads() == 256
+    with pytest.raises(E.InvalidArgument):
+        r.set_tile(8, 100)   # beyond the 40 KiB array
+    g = A.BlockIterativeReconstructionOperator(flags=E.ETN_RECON_FLAG_GENERIC_KERNELS, **members[3])
+    with pytest.raises(E.Unsupported):
+        g.set_workgroup_threads(256)
+    b = A.BatchReconstruction(members[:2])
+    b.run_iterations(1)
+    assert b.workgroup_threads(0) in (256, 512) and b.workgroup_threads(1) in (256, 512)
