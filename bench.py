@@ -1,7 +1,8 @@
 """SART throughput on one MI355X: GUPS on the 512^3 x 121-tilt workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxel-type float|half|bfloat16]
-                    [--workload metric|c2|c3|c4|c5] [--generic] [--mask] [--no-cpu-baseline]
+                    [--workload metric|c2|c3|c4|c5] [--generic] [--mask] [--fp-samples N]
+                    [--bp-samples M] [--no-cpu-baseline]
 
 A step is one full iteration of the block-iterative loop over the synthetic
 tilt series (for SART: every projection's forward projection, residual and
@@ -9,7 +10,9 @@ back projection).  GUPS = voxels x projections x steps / seconds / 1e9, with
 the volume, the measured projections and every buffer resident in HBM when
 the timed region starts.  Prints ONE JSON line on stdout; progress on stderr.
 --mask runs the metric and c2 workloads with a prior-knowledge mask (weight 1
-inside a y-axis cylinder holding the specimen, 0 outside).
+inside a y-axis cylinder holding the specimen, 0 outside); --fp-samples N
+(N x N sub-rays per pixel) and --bp-samples M (an M-point pattern per voxel)
+run them oversampled.
 """
 from __future__ import annotations
 
@@ -64,6 +67,8 @@ def main():
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--generic", action="store_true", help="force the generic (unfused) kernels")
     ap.add_argument("--mask", action="store_true", help="metric/c2: reconstruct with a prior-knowledge mask")
+    ap.add_argument("--fp-samples", type=int, default=1, help="metric/c2: forward sub-rays per pixel axis")
+    ap.add_argument("--bp-samples", type=int, default=1, help="metric/c2: back projection samples per voxel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-projections", type=int, default=32,
                     help="projection steps the CPU baseline times (about 0.35 s each on 16 threads)")
@@ -78,8 +83,8 @@ def main():
               "no exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    if args.mask and args.workload not in ("metric", "c2"):
-        print("--mask applies to the metric and c2 workloads", file=sys.stderr)
+    if (args.mask or args.fp_samples != 1 or args.bp_samples != 1) and args.workload not in ("metric", "c2"):
+        print("--mask, --fp-samples and --bp-samples apply to the metric and c2 workloads", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
         run_batch(args)
@@ -125,7 +130,8 @@ def main():
     r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs,
                                                algorithm=wl["algorithm"], block_size=wl["block_size"],
                                                lam=wl["lam"], projection_iteration=wl["order"],
-                                               voxel_type=args.voxel_type, flags=flags, mask=mask)
+                                               voxel_type=args.voxel_type, flags=flags, mask=mask,
+                                               fp_samples=args.fp_samples, bp_samples=args.bp_samples)
     fused = r.uses_fused_path()
     for _ in range(args.warmup):
         r.run_single_iteration()
@@ -148,8 +154,12 @@ def main():
 
     block = wl["block_size"] or (1 if wl["algorithm"] == "sart" else N)
     # the committed PMC traffic summaries cover the unmasked 512^3 metric workload only
+    plain = mask is None and args.fp_samples == 1 and args.bp_samples == 1
     roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block, masked=mask is not None,
-                               pmc=args.workload == "metric" and mask is None)
+                               pmc=args.workload == "metric" and plain)
+    if args.fp_samples != 1 or args.bp_samples != 1:
+        roofline["kernel"] += ", FP %dx%d sub-rays, BP %d samples" % (args.fp_samples, args.fp_samples,
+                                                                     args.bp_samples)
     del r
     torch.cuda.empty_cache()
 
@@ -163,8 +173,10 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, tilts, projs, args.cpu_sample_projections,
-                           None if mask is None else mask.cpu().numpy())
+        # oversampling multiplies the oracle's work per step by about (fp^2 + bp) / 2
+        cost = max(1.0, (args.fp_samples ** 2 + args.bp_samples) / 2.0)
+        cpu = cpu_baseline(wl, tilts, projs, max(1, int(round(args.cpu_sample_projections / cost))),
+                           None if mask is None else mask.cpu().numpy(), args.fp_samples, args.bp_samples)
     del mask
 
     out = {
@@ -190,6 +202,9 @@ def main():
     }
     if args.mask:
         out["config"]["mask"] = MASK_NAME
+    if args.fp_samples != 1 or args.bp_samples != 1:
+        out["config"]["fp_samples"] = args.fp_samples
+        out["config"]["bp_samples"] = args.bp_samples
     print(json.dumps(out), flush=True)
 
 
@@ -495,7 +510,7 @@ def measured_traffic(kernel, voxel_type):
     return traffic, os.path.relpath(path, ROOT), l2
 
 
-def cpu_baseline(wl, tilts, projs, nsample, mask=None):
+def cpu_baseline(wl, tilts, projs, nsample, mask=None, fp_samples=1, bp_samples=1):
     """The oracle (FP32-faithful C restatement of the reference's OpenCL path,
     OpenMP over the host cores) on a bounded sample of the same workload: the
     first `nsample` projection steps (FP + residual + BP) of a SART pass over
@@ -510,15 +525,17 @@ def cpu_baseline(wl, tilts, projs, nsample, mask=None):
     t0 = time.perf_counter()
     for i in range(nsample):
         g = O.rotate(base, float(tilts[i]))
-        p, l = O.forward_project(v, vol, g, mask=mask)
+        p, l = O.forward_project(v, vol, g, samples=fp_samples, mask=mask)
         res = O.compare(meas[i], p)
-        O.back_project(v, vol, g, res, l, lam=wl["lam"], mask=mask)
+        O.back_project(v, vol, g, res, l, lam=wl["lam"], samples=bp_samples, mask=mask)
     dt = time.perf_counter() - t0
     gups = nx * ny * nz * nsample / dt / 1e9
     log("cpu baseline: %d projection steps in %.1f s on %d threads -> %.4f GUPS" % (nsample, dt, O.num_threads(), gups))
     return {"value": round(gups, 5), "unit": "GUPS", "cores": O.num_threads(), "kind": "port",
-            "sample": "%d SART projection steps (FP+residual+BP) on the full %dx%dx%d volume, %d^2 detector%s"
-                      % (nsample, nx, ny, nz, nu, ", masked" if mask is not None else "")}
+            "sample": "%d SART projection steps (FP+residual+BP) on the full %dx%dx%d volume, %d^2 detector%s%s"
+                      % (nsample, nx, ny, nz, nu, ", masked" if mask is not None else "",
+                         ", FP %dx%d sub-rays, BP %d samples" % (fp_samples, fp_samples, bp_samples)
+                         if fp_samples != 1 or bp_samples != 1 else "")}
 
 
 if __name__ == "__main__":

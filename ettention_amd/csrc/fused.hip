@@ -38,6 +38,8 @@ constexpr int kMaxFusedFp = 16;      // forward projections per pass
 constexpr int kMaxFusedBp = 8;       // back projections per pass
 constexpr int kMaxRaysPerThread = 8;
 constexpr int kMaxPlanes = 8;        // planes per workgroup
+constexpr int kMaxFusedFpPerAxis = 4;   // forward sub-rays per pixel axis (oversampled kernels)
+constexpr int kMaxFusedBpSamples = 16;  // back projection samples per voxel (oversampled kernels)
 constexpr int kRowPad = 2;           // zero taps on each side of a staged detector row
 
 // Workgroup shapes.  512 threads with a 64 KiB LDS array run two workgroups
@@ -81,6 +83,10 @@ struct FusedBp {
     const float* residual;       // nv x nu
     const float* length;         // nv x nu
     float4 sample;               // single BP sample: offset xyz, in-voxel chord w
+    // oversampled kernels: the per_dir x per_dir sample pattern, y-major (the
+    // per_dir samples of one pattern row share a y offset)
+    int nsamples, per_dir;
+    float4 samples[kMaxFusedBpSamples];
 };
 
 struct FusedPass {
@@ -106,6 +112,9 @@ struct FusedVolume {
     const int* dead_rows;        // detector rows whose rays all miss the volume in y
     int ndead;
     const uint8_t* mask;         // prior-knowledge mask (complete volume, 255 = weight 1) or null: weight 1
+    int fp_n;                    // oversampled kernels: forward sub-rays per pixel axis
+    int fp_shift;                // log2 of the ray slots per pixel (a power of two >= fp_n: slot si traces sub-ray (si, 0))
+    int rowsets;                 // oversampled kernels: staged row pairs per plane (BP pattern rows)
     int* diag;                   // counts live rays found outside their plane (must stay 0)
 };
 
@@ -116,12 +125,15 @@ struct FusedWork {
 
 // LDS floats a workgroup of `threads` uses: the band (with a mask: the band,
 // at offset shape_band_floats(threads, true) the band's mask weights), then
-// (16-byte aligned) the staged detector records of each plane (row_floats per
-// plane: 8 floats per padded column with exact arithmetic, 4 with fast) and a
-// (m4 py, ty) pair per plane.
-__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats, bool mask, int threads)
+// (16-byte aligned) the staged detector records of each plane and row set
+// (row_floats per row set: 8 floats per padded column with exact arithmetic,
+// 4 with fast; one row set per BP pattern row) and a (m4 py, ty) pair per
+// plane and row set.
+__host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats, bool mask, int threads,
+                                                int rowsets)
 {
-    return (mask ? shape_band_floats(threads, true) : 0) + ((band * planes * nx + 3) & ~3) + planes * (row_floats + 2);
+    return (mask ? shape_band_floats(threads, true) : 0) + ((band * planes * nx + 3) & ~3) +
+           planes * rowsets * (row_floats + 2);
 }
 
 namespace {
@@ -314,9 +326,17 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // kMask applies the prior-knowledge mask (ImageHandler.hl:85-89): the ray
 // length sums dt * weight, the update is weight * sum.  A member without a
 // mask gets weight 1 (every update then equals the unmasked kernel's).
+// kOver (fast arithmetic) oversamples: fp_n x fp_n sub-rays per detector pixel
+// (ForwardProjectionKernel.cpp:65-68), whose sums the pixel averages, and a
+// per_dir x per_dir sample pattern per voxel (ParallelBeamsBackProjectionKernel
+// .cpp:13-103), each pattern row with its own staged row pair.  Sub-rays (si,
+// sj) of one si differ only in y (the detector's v axis is y, checked on the
+// host) and lie in the same plane, so they take the same x-z steps through the
+// same voxels and give bit-identical sums: the kernel traces sub-ray (si, 0)
+// and adds its sums fp_n times, in the generic kernel's (si, sj) order.
 // With at most two rays per thread the kernel fits 128 VGPRs: four waves per
 // SIMD, two 512-thread workgroups per CU (as LDS also allows).
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, int kThreads, int RPT>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT>
 __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
@@ -339,12 +359,13 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     const int width = np * nx;                          // one band row: the x rows of the np planes
     const int row_len = fv.row_len, pad_l = fv.pad_l;
     float* band = lds;                                  // band x width
-    // per plane row_len column records, 16-byte aligned after the band, then
-    // per plane (m4 py, ty) of the current back projection
+    // per plane and row set row_len column records, 16-byte aligned after the
+    // band, then per plane and row set (m4 py, ty) of the current back projection
+    const int rowsets = kOver ? fv.rowsets : 1;
     float* mband = lds + kMaskOff;                      // kMask: the band's mask weights
     float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskOff : 0) + ((fv.band * width + 3) & ~3));
     constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
-    float2* pterm = reinterpret_cast<float2*>(recs + np * kRecStride * row_len);
+    float2* pterm = reinterpret_cast<float2*>(recs + np * rowsets * kRecStride * row_len);
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
     const ETN_GLOBAL uint8_t* mask = kMask ? global_ptr(fv.mask) : nullptr;
     // MaskVolume weight of a byte, as the generic kernels compute it
@@ -355,22 +376,27 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
 
     const int nfp = pass->nfp, nbp = pass->nbp;
     const int R = fv.rows_per_plane;
-    const int rays_per_proj = np * R * nu;
+    const int fp_n = kOver ? fv.fp_n : 1;
+    const int fp_shift = kOver ? fv.fp_shift : 0;
+    const int rays_per_proj = (np * R * nu) << fp_shift;
     const int nrays = nfp * rays_per_proj;
     const SlabParams s = fv.slab;
     const ETN_GLOBAL int* row_lo = global_ptr(fv.row_lo);
     const ETN_GLOBAL int* row_hi = global_ptr(fv.row_hi);
 
-    // ray slot id = ((q * np + p) * R + r) * nu + u: projection q, plane j + p,
-    // its r-th detector row w, column u; slots past a plane's rows stay idle
-    auto decode = [&](int id, int& q, int& p, int& w, int& u) {
+    // ray slot id = (((q * np + p) * R + r) * nu + u) * 2^fp_shift + si:
+    // projection q, plane j + p, its r-th detector row w, column u, sub-ray
+    // column si; slots past a plane's rows or past fp_n stay idle
+    auto decode = [&](int id, int& q, int& p, int& w, int& u, int& sub) {
         q = id / rays_per_proj;
-        const int rem = id - q * rays_per_proj;
+        int rem = id - q * rays_per_proj;
+        sub = rem & ((1 << fp_shift) - 1);
+        rem >>= fp_shift;
         const int pr = rem / nu;
         u = rem - pr * nu;
         p = pr / R;
         w = row_lo[j + p] + (pr - p * R);
-        return w < row_hi[j + p];
+        return w < row_hi[j + p] && sub < fp_n;
     };
 
     // ---- ray set-up: dda_begin exactly as the generic FP kernel ----
@@ -382,8 +408,8 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         ray[k].acc = 0.0f;
         ray[k].accl = 0.0f;
         ray[k].off = 0;
-        int q, p, w, u;
-        if (id < nrays && decode(id, q, p, w, u)) {
+        int q, p, w, u, sub;
+        if (id < nrays && decode(id, q, p, w, u, sub)) {
             const ETN_GLOBAL FusedFp& f = pass->fp[q];
             RayParams g;
             for (int a = 0; a < 3; ++a) {
@@ -394,7 +420,13 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
             }
             const DdaConst c = dda_const(g.ray, s.vs);
             float src[3];
-            fp_ray_source(g, (uint32_t)u, (uint32_t)w, 0, 0, 1.0f, 0.5f, src);
+            if (kOver) {
+                // the generic FP's sub-ray loop: sd = 1 / n, offset sd / 2; sub-ray (si, 0)
+                const float sd = 1.0f / (float)fp_n;
+                fp_ray_source(g, (uint32_t)u, (uint32_t)w, (uint32_t)sub, 0, sd, sd / 2.0f, src);
+            } else {
+                fp_ray_source(g, (uint32_t)u, (uint32_t)w, 0, 0, 1.0f, 0.5f, src);
+            }
             DdaRay r;
             dda_begin(s, c, src, r);
             if (r.live && r.vox[1] != j + p) {
@@ -417,12 +449,12 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
 
     const int nbands = (nz + fv.band - 1) / fv.band;
 
-    // v coordinate of plane j + p under back projection bp, constant over the
-    // plane (checked on the host): returns detector row j0, sets the weight ty
-    // and the plane's voxel-centre y
-    auto plane_v = [&](const ETN_GLOBAL FusedBp& bp, int p, float& py, float& ty) {
+    // v coordinate of plane j + p under back projection bp for a sample with y
+    // offset dy, constant over the plane (checked on the host): returns
+    // detector row j0, sets the weight ty and the plane's voxel-centre y
+    auto plane_v = [&](const ETN_GLOBAL FusedBp& bp, int p, float dy, float& py, float& ty) {
         py = s.bmin[1] + ((float)(j + p) + 0.5f) * s.vs[1];
-        float vp = bp.m[1] * 0.0f + bp.m[5] * (py + bp.sample.y);
+        float vp = bp.m[1] * 0.0f + bp.m[5] * (py + dy);
         vp = vp + bp.m[9] * 0.0f;
         vp = vp + bp.m[13] * 1.0f;
         float base_row;
@@ -433,10 +465,14 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         const ETN_GLOBAL FusedBp& bp = pass->bp[q];
         const ETN_GLOBAL float* res = global_ptr(bp.residual);
         const ETN_GLOBAL float* len = global_ptr(bp.length);
-        for (int e = tid; e < np * row_len; e += kThreads) {
-            const int p = e / row_len, k = e - p * row_len, u = k - pad_l;
+        // record e = (p * rowsets + ys) * row_len + k: plane p, pattern row ys, column k
+        const int per_plane = rowsets * row_len;
+        for (int e = tid; e < np * per_plane; e += kThreads) {
+            const int p = e / per_plane, rem = e - p * per_plane, ys = rem / row_len;
+            const int k = rem - ys * row_len, u = k - pad_l;
+            const float dy = kOver ? bp.samples[ys * bp.per_dir].y : bp.sample.y;
             float py, ty;
-            const int row = plane_v(bp, p, py, ty);
+            const int row = plane_v(bp, p, dy, py, ty);
             float v[8];   // (value, difference) of residual row, row + 1, length row, row + 1
             for (int rr = 0; rr < 2; ++rr) {
                 const bool rok = row + rr >= 0 && row + rr < fv.nv;
@@ -460,10 +496,12 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                 recs[2 * e + 1] = make_float4(v[4], v[5], v[6], v[7]);
             }
         }
-        if (tid < np) {
+        if (tid < np * rowsets) {
+            const int p = tid / rowsets, ys = tid - p * rowsets;
+            const float dy = kOver ? bp.samples[ys * bp.per_dir].y : bp.sample.y;
             float py, ty;
-            plane_v(bp, tid, py, ty);
-            pterm[tid] = make_float2(bp.m[4] * (py + bp.sample.y), ty);
+            plane_v(bp, p, dy, py, ty);
+            pterm[tid] = make_float2(bp.m[4] * (py + dy), ty);
         }
     };
 
@@ -614,7 +652,53 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                 return S::round(value);
             };
             const int nrow = z1 - z0;
-            if (kThreads % gpr == 0) {
+            if constexpr (kOver) {
+                // the sample loop of ParallelBeamsBackProjection.cl:26-52 over the
+                // pattern; pattern row ys reads its own staged row pair
+                const int nsmp = bp.nsamples, pd = bp.per_dir;
+#pragma unroll 1
+                for (int gi = tid; gi < ng; gi += kThreads) {
+                    const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
+                    int p = 0, x0 = xw;
+                    if (np > 1) {
+                        p = xw / nx;
+                        x0 = xw - p * nx;
+                    }
+                    const float cz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2];
+                    float out[G];
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        const float cx = s.bmin[0] + ((float)(x0 + i) + 0.5f) * s.vs[0];
+                        float sum = 0.0f;
+                        int used = 0;
+#pragma unroll 1
+                        for (int t = 0; t < nsmp; ++t) {
+                            const float4 sm = bp.samples[t];
+                            const int ys = t / pd;
+                            const float2 pt = pterm[p * rowsets + ys];
+                            float u = m0 * (cx + sm.x) + pt.x;
+                            u = u + m8 * (cz + sm.z);
+                            u = u + m12 * 1.0f;
+                            u = u - 0.5f;
+                            const float bu = floorf(u);
+                            const float tx = u - bu;
+                            const float4 c = recs[(p * rowsets + ys) * row_len + (int)bu + pad_l];
+                            const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
+                            const float contrib = Rv * (lam * sm.w) * __builtin_amdgcn_rcpf(L);
+                            const bool ok = L > kEpsilon;
+                            sum += ok ? contrib : 0.0f;
+                            used += ok ? 1 : 0;
+                        }
+                        const int at = zz * width + xw + i;
+                        float value = band[at];
+                        if (used != 0) value += (kMask ? band[at + kMaskOff] * sum : sum) / (float)used;
+                        out[i] = S::round(value);
+                        band[at] = out[i];
+                    }
+                    if (q == nbp - 1)
+                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
+                }
+            } else if (kThreads % gpr == 0) {
                 // every thread keeps the same x group (and plane) in every band row
                 const int xw = (tid % gpr) * G, dz = kThreads / gpr;
                 int p = 0, x0 = xw;
@@ -699,17 +783,40 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     }
 
     // ---- residual (+ long-object compensation) per pixel ----
+    // oversampled: every traced sub-ray's sums go through LDS (free after the
+    // last band); the pixel's first slot adds sub-ray (si, sj)'s sums, those of
+    // (si, 0), for si, then sj, as the generic kernel does
+    float2* part = reinterpret_cast<float2*>(lds);
+    if (kOver) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int id = k * kThreads + tid;
+            if (id < nrays) part[id] = make_float2(ray[k].acc, ray[k].accl);
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int id = k * kThreads + tid;
-        int q, p, w, u;
-        if (id < nrays && decode(id, q, p, w, u)) {
+        int q, p, w, u, sub;
+        if (id < nrays && decode(id, q, p, w, u, sub) && sub == 0) {
             const ETN_GLOBAL FusedFp& f = pass->fp[q];
             float acc = 0.0f, accl = 0.0f;
-            acc += ray[k].acc;
-            accl += ray[k].accl;
-            float pv = 0.0f + acc / 1.0f;
-            float l = 0.0f + accl / 1.0f;
+            if (kOver) {
+                for (int si = 0; si < fp_n; ++si) {
+                    const float2 pa = part[id + si];
+                    for (int sj = 0; sj < fp_n; ++sj) {
+                        acc += pa.x;
+                        accl += pa.y;
+                    }
+                }
+            } else {
+                acc += ray[k].acc;
+                accl += ray[k].accl;
+            }
+            const float total = (float)(fp_n * fp_n);
+            float pv = 0.0f + acc / total;
+            float l = 0.0f + accl / total;
             if (f.use_loc) {
                 pv = l > kEpsilon ? pv * f.loc_c / l : 0.0f;
                 l = f.loc_c;
@@ -751,57 +858,73 @@ struct FusedLaunch {
     size_t lds;
 };
 
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, int kThreads>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads>
 void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
 {
     if (l.lds > sizeof(float) * (size_t)FusedShape<kThreads>::kLds) fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kThreads);
     switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kThreads, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
     }
 }
 
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask>
+// The oversampled kernels exist for 512-thread workgroups only.
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver>
 void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 {
-    if (l.threads == 256) launch_fused_shape<S, kVec, kCap, kFast, kMask, 256>(l, stream);
-    else if (l.threads == 512) launch_fused_shape<S, kVec, kCap, kFast, kMask, 512>(l, stream);
-    else fail("fused plane kernel: workgroups have 256 or 512 threads");
+    if (l.threads == 512) {
+        launch_fused_shape<S, kVec, kCap, kFast, kMask, kOver, 512>(l, stream);
+        return;
+    }
+    if constexpr (!kOver) {
+        if (l.threads == 256) {
+            launch_fused_shape<S, kVec, kCap, kFast, kMask, false, 256>(l, stream);
+            return;
+        }
+    }
+    fail("fused plane kernel: workgroups have 512 threads, or 256 without oversampling");
 }
 
-template <typename S, bool kFast, bool kMask>
+template <typename S, bool kFast, bool kMask, bool kOver>
 void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
-    if (vec && !cap) launch_fused_rpt<S, true, false, kFast, kMask>(l, stream);
-    else if (vec) launch_fused_rpt<S, true, true, kFast, kMask>(l, stream);
-    else if (!cap) launch_fused_rpt<S, false, false, kFast, kMask>(l, stream);
-    else launch_fused_rpt<S, false, true, kFast, kMask>(l, stream);
+    if (vec && !cap) launch_fused_rpt<S, true, false, kFast, kMask, kOver>(l, stream);
+    else if (vec) launch_fused_rpt<S, true, true, kFast, kMask, kOver>(l, stream);
+    else if (!cap) launch_fused_rpt<S, false, false, kFast, kMask, kOver>(l, stream);
+    else launch_fused_rpt<S, false, true, kFast, kMask, kOver>(l, stream);
+}
+
+template <bool kMask, bool kOver>
+void launch_fused_fast(etn_voxel_type vt, bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
+{
+    if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true, kMask, kOver>(vec, cap, l, stream);
+    else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true, kMask, kOver>(vec, cap, l, stream);
+    else launch_fused_typed<BF16, true, kMask, kOver>(vec, cap, l, stream);
 }
 
 }  // namespace
 
 // vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049;
 // exact: the generic kernels' operations and rounding order; mask: the launch
-// carries prior-knowledge masks (fast arithmetic only)
-void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask, const FusedLaunch& l,
+// carries prior-knowledge masks; over: oversampled FP and BP (mask and over
+// with fast arithmetic only)
+void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask, bool over, const FusedLaunch& l,
                   hipStream_t stream)
 {
     if (exact) {
-        if (mask) unsupported("fused plane kernel: a mask needs fast arithmetic");
-        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, false>(vec, cap, l, stream);
-        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, false>(vec, cap, l, stream);
-        else launch_fused_typed<BF16, false, false>(vec, cap, l, stream);
-    } else if (mask) {
-        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true, true>(vec, cap, l, stream);
-        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true, true>(vec, cap, l, stream);
-        else launch_fused_typed<BF16, true, true>(vec, cap, l, stream);
+        if (mask || over) unsupported("fused plane kernel: a mask or oversampling needs fast arithmetic");
+        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, false, false>(vec, cap, l, stream);
+        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, false, false>(vec, cap, l, stream);
+        else launch_fused_typed<BF16, false, false, false>(vec, cap, l, stream);
+    } else if (over) {
+        if (mask) launch_fused_fast<true, true>(vt, vec, cap, l, stream);
+        else launch_fused_fast<false, true>(vt, vec, cap, l, stream);
     } else {
-        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, true, false>(vec, cap, l, stream);
-        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, true, false>(vec, cap, l, stream);
-        else launch_fused_typed<BF16, true, false>(vec, cap, l, stream);
+        if (mask) launch_fused_fast<true, false>(vt, vec, cap, l, stream);
+        else launch_fused_fast<false, false>(vt, vec, cap, l, stream);
     }
     hip_check(hipGetLastError(), "fused plane kernel launch");
 }
@@ -831,6 +954,9 @@ struct MemberPlan {
     int row_floats = 0;                // LDS floats of one plane's staged rows
     const uint8_t* mask = nullptr;     // prior-knowledge mask (device), or null
     int threads = dev::kMaxThreads;    // workgroup shape of the engine's launches
+    int fp_n = 1, fp_shift = 0;        // forward sub-rays per pixel axis; log2 of ray slots (columns si) per pixel
+    int bp_samples = 1, rowsets = 1;   // back projection samples per voxel; pattern rows
+    bool oversampled() const { return fp_n != 1 || bp_samples != 1; }
     int ndead = 0;
     uint32_t bmax = 0;                 // largest block
     DeviceBuffer rows;                 // row_lo[ny] | row_hi[ny] | dead rows
@@ -851,11 +977,12 @@ int select_rpt(int rays, int threads)
 int max_band(const MemberPlan& m, int planes, int threads)
 {
     if (planes < 1 || planes > dev::kMaxPlanes || (planes & (planes - 1)) || m.ny % planes) return 0;
-    if (planes * m.rows_per_plane * m.nu > threads * dev::kMaxRaysPerThread) return 0;
+    if ((planes * m.rows_per_plane * m.nu << m.fp_shift) > threads * dev::kMaxRaysPerThread) return 0;
+    if (m.oversampled() && threads != dev::kMaxThreads) return 0;   // oversampled kernels: 512 threads only
     const bool mask = m.mask != nullptr;
     const int masked = mask ? dev::shape_band_floats(threads, true) : 0;
-    int band = std::min(m.nz, (dev::shape_lds_floats(threads) - masked - 3 - planes * (m.row_floats + 2)) /
-                                  (planes * m.nx));
+    int band = std::min(m.nz, (dev::shape_lds_floats(threads) - masked - 3 -
+                               planes * m.rowsets * (m.row_floats + 2)) / (planes * m.nx));
     band = std::min(band, dev::shape_band_floats(threads, mask) / (planes * m.nx));
     return std::max(band, 0);
 }
@@ -865,7 +992,7 @@ int max_band(const MemberPlan& m, int planes) { return max_band(m, planes, m.thr
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
-           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr, m.threads) <=
+           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr, m.threads, m.rowsets) <=
                dev::shape_lds_floats(m.threads);
 }
 
@@ -877,7 +1004,7 @@ Tile default_tile(const MemberPlan& m)
         const int b = max_band(m, p);
         if (b < 1) continue;
         best = {p, b};
-        if (p * m.rows_per_plane * m.nu >= m.threads) break;
+        if ((p * m.rows_per_plane * m.nu << m.fp_shift) >= m.threads) break;
     }
     return best;
 }
@@ -915,7 +1042,20 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     if (exact && op.getReconstructedVolume()->properties().subvolume_count() != 1) return false;
     const VolumeProperties whole = whole_volume(op.getReconstructedVolume()->properties());
     const VolumeProperties& vp = whole;
-    if ((d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1) return false;
+    // oversampling: up to 4 x 4 forward sub-rays and a 4 x 4 back projection
+    // pattern, with fast arithmetic (the exact kernels take one sample each)
+    const int fp_n = (int)(d.fp_samples ? d.fp_samples : 1);
+    const int bp_s = (int)(d.bp_samples ? d.bp_samples : 1);
+    const int bp_pd = (int)std::lround(std::sqrt((double)bp_s));
+    if (fp_n > dev::kMaxFusedFpPerAxis || bp_s > dev::kMaxFusedBpSamples || bp_pd * bp_pd != bp_s) return false;
+    if (exact && (fp_n != 1 || bp_s != 1)) return false;
+    // forward sub-rays restarted at every slab face (the reference's slab-wise
+    // FP) differ from one sweep by up to 2.4e-4 relative L2 after two SART
+    // iterations (4 x 4 sub-rays, 2 or 4 slabs): beyond the FP32 bound, so an
+    // oversampled FP over slabs stays on the generic kernels
+    if (fp_n != 1 && op.getReconstructedVolume()->properties().subvolume_count() != 1) return false;
+    int fp_shift = 0;
+    while ((1 << fp_shift) < fp_n) ++fp_shift;     // ray slots per pixel: sub-ray columns si
     if (d.mask && exact) return false;   // the exact kernels carry no mask: generic kernels
     if (reinterpret_cast<uintptr_t>(d.mask) % 4) return false;   // masks are read 4 bytes at a time
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
@@ -928,6 +1068,8 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
             const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
             const Vec3f r = g.ray_direction();
             if (r.y != 0.0f || g.horizontal_pitch().y != 0.0f) return false;
+            // oversampled FP: sub-rays of one column differ only in y
+            if (fp_n > 1 && (g.vertical_pitch().x != 0.0f || g.vertical_pitch().z != 0.0f)) return false;
             // nonzero x/z components must have finite reciprocals (the plane
             // DDA relies on every res value staying at or below FLT_MAX)
             for (float c : {r.x, r.z})
@@ -936,28 +1078,35 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
             const float* mv = m.data();
             if (mv[1] != 0.0f || mv[9] != 0.0f || mv[3] != 0.0f || mv[7] != 0.0f || mv[11] != 0.0f || mv[15] != 1.0f)
                 return false;
-            // plane of each detector row, with the FP's own arithmetic (samples = 1);
-            // -1 for rows whose sources lie outside the box in y by a margin that
-            // makes every ray's y slab interval empty (t_out <= -2^-20 * FLT_MAX)
+            // plane of each detector row, with the FP's own arithmetic for every
+            // sub-ray row sj (all must agree); -1 for rows whose sources lie
+            // outside the box in y by a margin that makes every ray's y slab
+            // interval empty (t_out <= -2^-20 * FLT_MAX)
             std::vector<int> map(nv);
             const SubVolume& sub = vp.subvolume(0);
             const float ivs = 1.0f / vp.voxel_size().y;
             const float margin = 1.0f / 1048576.0f;
+            const float sd = fp_n == 1 ? 1.0f : 1.0f / (float)fp_n;
+            const float off = fp_n == 1 ? 0.5f : sd / 2.0f;
             for (int w = 0; w < nv; ++w) {
-                const float fx = (float)0 + 0.5f + 1.0f * 0.0f;
-                const float fy = (float)w + 0.5f + 1.0f * 0.0f;
-                const float ru = g.horizontal_pitch().y * fx, rv = g.vertical_pitch().y * fy;
-                float y = g.source_base().y + ru + rv;
-                if (sub.bmin.y - y >= margin || y - sub.bmax.y >= margin) {
-                    map[w] = -1;
-                    continue;
+                for (int sj = 0; sj < fp_n; ++sj) {
+                    const float fx = (float)0 + off + sd * 0.0f;
+                    const float fy = (float)w + off + sd * (float)sj;
+                    const float ru = g.horizontal_pitch().y * fx, rv = g.vertical_pitch().y * fy;
+                    float y = g.source_base().y + ru + rv;
+                    int jj;
+                    if (sub.bmin.y - y >= margin || y - sub.bmax.y >= margin) {
+                        jj = -1;
+                    } else {
+                        if (y < sub.bmin.y || y > sub.bmax.y) return false;   // too close to call: generic kernels
+                        y = std::fmin(std::fmax(y, sub.bmin.y), sub.bmax.y);
+                        jj = (int)((y - sub.bmin.y) * ivs);
+                        jj = std::max(jj, 0);
+                        jj = std::min(jj, ny - 1);
+                    }
+                    if (sj == 0) map[w] = jj;
+                    else if (map[w] != jj) return false;   // the pixel's sub-rays cross planes
                 }
-                if (y < sub.bmin.y || y > sub.bmax.y) return false;   // too close to call: generic kernels
-                y = std::fmin(std::fmax(y, sub.bmin.y), sub.bmax.y);
-                int jj = (int)((y - sub.bmin.y) * ivs);
-                jj = std::max(jj, 0);
-                jj = std::min(jj, ny - 1);
-                map[w] = jj;
             }
             if (plane_of_row.empty()) plane_of_row = map;
             else if (plane_of_row != map) return false;
@@ -981,7 +1130,8 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     }
     int rows_per_plane = 0;
     for (int jj = 0; jj < ny; ++jj) rows_per_plane = std::max(rows_per_plane, hi[jj] - lo[jj]);
-    if (rows_per_plane < 1 || rows_per_plane * nu > dev::kMaxThreads * dev::kMaxRaysPerThread) return false;
+    if (rows_per_plane < 1 || (rows_per_plane * nu << fp_shift) > dev::kMaxThreads * dev::kMaxRaysPerThread)
+        return false;
 
     out.op = &op;
     out.nx = nx;
@@ -1005,15 +1155,15 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
                 const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
                 const Matrix4x4 pm = g.projection_matrix();
                 const float* mv = pm.data();
-                const std::vector<VoxelSample> smp = bp_sample_pattern(g, vs, 1);
-                for (int c = 0; c < 8; ++c) {
-                    const double px = ((c & 1) ? hi.x - 0.5 * vs.x : lo.x + 0.5 * vs.x) + smp[0].dx;
-                    const double py = ((c & 2) ? hi.y - 0.5 * vs.y : lo.y + 0.5 * vs.y) + smp[0].dy;
-                    const double pz = ((c & 4) ? hi.z - 0.5 * vs.z : lo.z + 0.5 * vs.z) + smp[0].dz;
-                    const double u = mv[0] * px + mv[4] * py + mv[8] * pz + mv[12];
-                    umin = std::min(umin, u);
-                    umax = std::max(umax, u);
-                }
+                for (const VoxelSample& smp : bp_sample_pattern(g, vs, (uint32_t)bp_s))
+                    for (int c = 0; c < 8; ++c) {
+                        const double px = ((c & 1) ? hi.x - 0.5 * vs.x : lo.x + 0.5 * vs.x) + smp.dx;
+                        const double py = ((c & 2) ? hi.y - 0.5 * vs.y : lo.y + 0.5 * vs.y) + smp.dy;
+                        const double pz = ((c & 4) ? hi.z - 0.5 * vs.z : lo.z + 0.5 * vs.z) + smp.dz;
+                        const double u = mv[0] * px + mv[4] * py + mv[8] * pz + mv[12];
+                        umin = std::min(umin, u);
+                        umax = std::max(umax, u);
+                    }
             }
         if (!(umin > -1e6 && umax < 1e6)) return false;
         const int bu_lo = (int)std::floor(umin - 0.5) - 1, bu_hi = (int)std::floor(umax - 0.5) + 1;
@@ -1023,7 +1173,11 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     out.row_floats = out.row_len * (exact ? 8 : 4);
     out.mask = d.mask;
     out.threads = dev::kMaxThreads;
-    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr, out.threads) >
+    out.fp_n = fp_n;
+    out.fp_shift = fp_shift;
+    out.bp_samples = bp_s;
+    out.rowsets = bp_pd;
+    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr, out.threads, out.rowsets) >
         dev::shape_lds_floats(out.threads))
         return false;   // a z row and the rows fit
     if (max_band(out, 1) < 1) return false;   // not even one z row of one plane fits
@@ -1047,7 +1201,8 @@ void build_passes(MemberPlan& mb)
     BlockIterativeReconstructionOperator& op = *mb.op;
     mb.passes.clear();
     const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, mb.threads * dev::kMaxRaysPerThread /
-                                                                  (mb.tile.planes * mb.rows_per_plane * mb.nu)));
+                                                                  ((mb.tile.planes * mb.rows_per_plane * mb.nu)
+                                                                   << mb.fp_shift)));
     auto image = [&](uint32_t parity, uint32_t slot, int which) {
         return mb.images.as<float>() + mb.pixels * ((((size_t)parity * mb.bmax + slot) * 2) + which);
     };
@@ -1112,8 +1267,12 @@ void build_passes(MemberPlan& mb)
             e.lambda = mb.lambda / (float)bs;
             e.residual = image(parity, p, 0);
             e.length = image(parity, p, 1);
-            const std::vector<VoxelSample> smp = bp_sample_pattern(g, vp.voxel_size(), 1);
+            const std::vector<VoxelSample> smp = bp_sample_pattern(g, vp.voxel_size(), (uint32_t)mb.bp_samples);
             e.sample = make_float4(smp[0].dx, smp[0].dy, smp[0].dz, smp[0].length);
+            e.nsamples = (int)smp.size();
+            e.per_dir = mb.rowsets;
+            for (size_t k = 0; k < smp.size(); ++k)
+                e.samples[k] = make_float4(smp[k].dx, smp[k].dy, smp[k].dz, smp[k].length);
             pending.push_back(e);
         }
     }
@@ -1123,7 +1282,7 @@ void build_passes(MemberPlan& mb)
 class FusedEngineImpl final : public FusedPlaneEngine {
 public:
     FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask,
-                    hipStream_t stream);
+                    bool over, hipStream_t stream);
     void enqueueIteration(hipStream_t stream) override;
     void check(hipStream_t stream) override;
     std::vector<LaunchTime> timeIteration(hipStream_t stream) override;
@@ -1159,6 +1318,7 @@ private:
     etn_voxel_type vt_;
     bool vec_, cap_, exact_;
     bool mask_;                         // every member has a mask (the launches use the mask kernels)
+    bool over_;                         // every member oversamples (the launches use the oversampled kernels)
     int threads_ = dev::kMaxThreads;    // workgroup shape of every launch
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
@@ -1166,8 +1326,8 @@ private:
 };
 
 FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
-                                 bool mask, hipStream_t stream)
-    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact), mask_(mask)
+                                 bool mask, bool over, hipStream_t stream)
+    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact), mask_(mask), over_(over)
 {
     diag_.allocate(sizeof(int));
     ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
@@ -1205,6 +1365,9 @@ void FusedEngineImpl::upload(hipStream_t stream)
         fv.dead_rows = m.rows.as<int>() + 2 * m.ny;
         fv.ndead = m.ndead;
         fv.mask = m.mask;
+        fv.fp_n = m.fp_n;
+        fv.fp_shift = m.fp_shift;
+        fv.rowsets = m.rowsets;
         fv.diag = diag_.as<int>();
         vols.push_back(fv);
     }
@@ -1281,9 +1444,11 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
             const dev::FusedPass& p = m.passes[pi];
             tables[k * members_.size() + i] = base + pass_base_[i] + pi;
             for (int j = 0; j < m.ny; j += m.tile.planes) work.push_back({(int)i, j});
-            l.rpt = std::max(l.rpt, select_rpt(p.nfp * m.tile.planes * m.rows_per_plane * m.nu, threads_));
+            l.rpt = std::max(l.rpt, select_rpt((p.nfp * m.tile.planes * m.rows_per_plane * m.nu) << m.fp_shift,
+                                               threads_));
             l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx,
-                                                                                   m.row_floats, mask_, threads_));
+                                                                                   m.row_floats, mask_, threads_,
+                                                                                   m.rowsets));
             l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
         }
         l.groups = (int)(work.size() - l.work);
@@ -1310,7 +1475,7 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         const dev::FusedLaunch fl{threads_, l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
                                   tables + k * members_.size(), l.lds};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
-        dev::launch_fused(vt_, vec_, cap_, exact_, mask_, fl, stream);
+        dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
     }
 }
@@ -1412,10 +1577,12 @@ float FusedEngineImpl::timeLaunches(const Schedule& s, hipStream_t stream) const
 // current workgroup shape.
 void FusedEngineImpl::tuneTiles(hipStream_t stream)
 {
-    std::map<std::tuple<int, int, int, int, int, int>, std::vector<size_t>> classes;
+    // (the staged row length and row sets and the ray slots per pixel decide
+    // which tiles fit as much as the sizes do)
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int>, std::vector<size_t>> classes;
     for (size_t i = 0; i < members_.size(); ++i) {
         const MemberPlan& m = members_[i];
-        classes[{m.nx, m.ny, m.nz, m.nu, m.nv, m.rows_per_plane}].push_back(i);
+        classes[{m.nx, m.ny, m.nz, m.nu, m.nv, m.rows_per_plane, m.row_floats, m.rowsets, m.fp_shift}].push_back(i);
     }
     for (auto& kv : classes) {
         const std::vector<size_t>& ids = kv.second;
@@ -1502,10 +1669,11 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
     taken->assign(members.size(), false);
     std::vector<MemberPlan> plans;
     etn_voxel_type vt = ETN_VOXEL_FLOAT32;
-    bool vec = true, cap = false, mask = false;
+    bool vec = true, cap = false, mask = false, over = false;
     for (size_t i = 0; i < members.size(); ++i) {
         const etn_reconstruction_desc& d = *members[i].desc;
-        if (!plans.empty() && (d.voxel_type != vt || (d.mask != nullptr) != mask ||
+        const bool d_over = (d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1;
+        if (!plans.empty() && (d.voxel_type != vt || (d.mask != nullptr) != mask || d_over != over ||
                                (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
             continue;
         MemberPlan m;
@@ -1514,13 +1682,14 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
             vt = d.voxel_type;
             vec = m.nx % 4 == 0;
             mask = m.mask != nullptr;
+            over = m.oversampled();
         }
         cap = cap || m.nx + m.nz > 2049;
         plans.push_back(std::move(m));
         (*taken)[i] = true;
     }
     if (plans.empty()) return nullptr;
-    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, exact, mask, stream);
+    return std::make_unique<FusedEngineImpl>(std::move(plans), vt, vec, cap, exact, mask, over, stream);
 }
 
 }  // namespace etn

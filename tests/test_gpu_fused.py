@@ -109,7 +109,10 @@ def test_ineligible_configurations_use_generic_kernels(synthetic):
     gt, tilts, projs = synthetic
     sub = slice(None, None, 20)
     p = dev(projs[sub])
-    for kw in (dict(fp_samples=4), dict(bp_samples=4),
+    for kw in (dict(fp_samples=4, flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
+               dict(bp_samples=4, flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
+               dict(fp_samples=5), dict(bp_samples=25),
+               dict(fp_samples=2, volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=4)),
                dict(volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=2), flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
                dict(x_axis_tilt=5.0),
                dict(mask=torch.full((64, 64, 64), 255, dtype=torch.uint8, device="cuda"),
@@ -259,3 +262,91 @@ def test_full_weight_mask_equals_no_mask_on_fused_path(synthetic):
     b.run()
     assert torch.equal(a.get_reconstructed_volume(), b.get_reconstructed_volume())
     assert a.residual_norm() == b.residual_norm()
+
+
+@pytest.mark.parametrize("case", [
+    # the reference's synthetic_3_parallel/sart.xml without its slab split: FP samples 4, BP samples 9, lambda 0.1
+    dict(shape=(64, 64, 64), geometry="test", algorithm="sart", lam=0.1, fp_samples=4, bp_samples=9,
+         projection_iteration="random"),
+    # BP oversampling over 4 slabs (one sweep regroups nothing in the BP)
+    dict(shape=(64, 64, 64), geometry="test", subvolumes=4, algorithm="sart", lam=0.3, bp_samples=4,
+         projection_iteration="identity"),
+    # the masked_reconstruction files: FP and BP samples 4 with a mask
+    dict(shape=(64, 64, 64), geometry="test", algorithm="sart", lam=0.3, fp_samples=4, bp_samples=4, mask=True,
+         projection_iteration="identity"),
+    dict(shape=(66, 48, 40), geometry="mrc", algorithm="blockiterative", block_size=3, lam=0.5, fp_samples=2,
+         bp_samples=4, use_long_object_compensation=True, projection_iteration="maxangle"),
+    dict(shape=(64, 64, 64), geometry="test", algorithm="sirt", lam=1.0, fp_samples=3, bp_samples=1,
+         projection_iteration="identity"),
+    dict(shape=(64, 64, 64), geometry="test", algorithm="sart", lam=0.3, fp_samples=1, bp_samples=16,
+         projection_iteration="identity", voxel_type="bfloat16"),
+])
+def test_oversampling_runs_fused_with_fast_arithmetic(synthetic, case):
+    """FP sub-rays (fp_n x fp_n per pixel, averaged) and the BP sample pattern on
+    the fused kernels (default arithmetic).  Within 1e-5 of the oversampled
+    generic kernels and the oversampled oracle (2^-9 for bf16 storage, as for
+    the mask)."""
+    case = dict(case)
+    shape, geometry = case.pop("shape"), case.pop("geometry")
+    subvolumes = case.pop("subvolumes", 1)
+    use_mask = case.pop("mask", False)
+    nx, ny, nz = shape
+    vg = E.VolumeGeometry.make(shape, subvolumes=subvolumes)
+    if geometry == "test":
+        _, tilts, projs = synthetic
+        tilts, projs = tilts[::6], np.ascontiguousarray(projs[::6])
+        base = TEST_BASE
+    else:
+        base = E.ScannerGeometry.mrc_base(nx, ny)
+        tilts = S.tilt_series(60.0, 6.0)
+        truth_t = dev(S.ellipsoid_phantom((nz, ny, nx), seed=79))
+        pt = torch.zeros((len(tilts), ny, nx), dtype=torch.float32, device="cuda")
+        scratch = torch.zeros((ny, nx), dtype=torch.float32, device="cuda")
+        for i, t in enumerate(tilts):
+            scratch.zero_()
+            A.forward_project(E.VolumeGeometry.make(shape), truth_t, A.rotate(base, float(t)), pt[i], scratch)
+        projs = pt.cpu().numpy()
+    mask = seeded_mask(shape, 6) if use_mask else None
+    vols = []
+    for flags in (0, E.ETN_RECON_FLAG_GENERIC_KERNELS):
+        r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                                   projections=dev(projs), iterations=2, flags=flags,
+                                                   mask=None if mask is None else dev(mask), **case)
+        assert r.uses_fused_path() == (flags == 0)
+        r.run()
+        r.synchronize()
+        vols.append(r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy())
+    fast, generic = vols
+    f32 = case.get("voxel_type", "float") == "float"
+    assert rel_l2(fast, generic) <= (1e-5 if f32 else 2.0 ** -9)
+    if f32:
+        B = {"sart": 1, "sirt": 0xFFFFFFFF}.get(case["algorithm"], case.get("block_size", 1))
+        spec = O.ReconSpec(ov(vg), og(base), tilts, projs, block_size=B, order=case["projection_iteration"],
+                           lam=case["lam"], use_loc=case.get("use_long_object_compensation", False),
+                           fp_samples=case.get("fp_samples", 1), bp_samples=case.get("bp_samples", 1), mask=mask)
+        ref = spec.run(np.zeros((nz, ny, nx), np.float32), 2)
+        assert rel_l2(fast, ref) <= 1e-5
+
+
+def test_oversampled_members_share_their_own_engine():
+    """A batch mixing oversampled and single-sample members: each volume equals
+    the member run alone on the fused path, bit for bit."""
+    base = TEST_BASE
+    s = load_golden("synthetic_3_parallel")
+    tilts, projs = s["angles"][::9], dev(np.ascontiguousarray(s["projections"][::9]))
+    common = dict(volume=E.VolumeGeometry.make((64, 64, 64)), base_geometry=base, tilt_angles=tilts,
+                  projections=projs, algorithm="sart", lam=0.2, projection_iteration="identity")
+    members = [dict(common, fp_samples=2, bp_samples=4), dict(common), dict(common, fp_samples=4, bp_samples=9)]
+    b = A.BatchReconstruction(members)
+    assert all(b.uses_fused_path(i) for i in range(3))
+    b.run_iterations(2)
+    b.synchronize()
+    for i, m in enumerate(members):
+        r = A.BlockIterativeReconstructionOperator(**m)
+        r.run_iterations(2)
+        r.synchronize()
+        np.testing.assert_array_equal(b.get_reconstructed_volume(i, dtype=torch.float32).cpu().numpy(),
+                                      r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy())
+    o = A.BlockIterativeReconstructionOperator(**members[0])
+    with pytest.raises(E.InvalidArgument):
+        o.set_workgroup_threads(256)     # the oversampled kernels run 512-thread workgroups
