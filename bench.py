@@ -162,6 +162,11 @@ def main():
                                                                      args.bp_samples)
     del r
     torch.cuda.empty_cache()
+    # the achievable bandwidth on this box, and the dominant kernel's measured
+    # HBM traffic rate beside it
+    roofline["stream_copy_gbs"] = stream_copy_gbs(dev)
+    if roofline.get("traffic"):
+        roofline["traffic_gbs"] = round(roofline["traffic"] / (roofline["avg_launch_ms"] * 1e-3) / 1e9, 1)
 
     if args.calibrate_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -206,6 +211,26 @@ def main():
         out["config"]["fp_samples"] = args.fp_samples
         out["config"]["bp_samples"] = args.bp_samples
     print(json.dumps(out), flush=True)
+
+
+def stream_copy_gbs(dev, floats=1 << 28, reps=5):
+    """Device-to-device copy bandwidth (read + write bytes / time) of two 1 GiB
+    buffers, HIP events around `reps` copies after one untimed copy."""
+    import torch
+    src = torch.ones(floats, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    gbs = 2 * 4 * floats * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    log("stream copy: %.1f GB/s (1 GiB read + 1 GiB write per copy)" % gbs)
+    return round(gbs, 1)
 
 
 MASK_NAME = "prior-knowledge mask: 255 inside the y-axis cylinder of radius 0.45 nx centred in x-z, 0 outside"
