@@ -149,8 +149,10 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     gups = V * N * args.steps / elapsed / 1e9
     log("timed %d iterations: %.2f ms/iteration, %.1f GUPS" % (args.steps, ms_per_step, gups))
+    t_norm = time.perf_counter()
     norm = r.residual_norm()
-    log("residual norm after %d iterations: %.6e" % (args.warmup + args.steps, norm))
+    log("residual norm after %d iterations: %.6e (FP64, %.1f ms)"
+        % (args.warmup + args.steps, norm, 1e3 * (time.perf_counter() - t_norm)))
 
     block = wl["block_size"] or (1 if wl["algorithm"] == "sart" else N)
     # the committed PMC traffic summaries cover the unmasked 512^3 metric workload only

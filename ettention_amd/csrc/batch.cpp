@@ -1,4 +1,5 @@
 #include "batch.hpp"
+#include "kernels.hpp"
 
 namespace etn {
 
@@ -86,6 +87,17 @@ void BatchReconstruction::tile(uint32_t i, int* planes, int* band) const
 {
     if (!usesFusedPath(i)) unsupported("this batch member does not use the fused plane kernels");
     engines_[slots_[i].engine]->tile(slots_[i].index, planes, band);
+}
+
+double BatchReconstruction::residualNorm(uint32_t i)
+{
+    if (i >= ops_.size()) fail("batch member index out of range");
+    if (slots_[i].engine < 0) return ops_[i]->residualNorm();
+    if (!norm_scratch_.get()) norm_scratch_.allocate(sizeof(double) * (dev::kSsdBlocks + 1));
+    FusedPlaneEngine& e = *engines_[slots_[i].engine];
+    const double norm = e.residualNorm(slots_[i].index, norm_scratch_.as<double>(), stream_);
+    e.check(stream_);
+    return norm;
 }
 
 int BatchReconstruction::workgroupThreads(uint32_t i) const

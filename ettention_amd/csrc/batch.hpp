@@ -23,6 +23,8 @@ public:
     bool usesFusedPath(uint32_t i) const;
     void tile(uint32_t i, int* planes, int* band) const;
     int workgroupThreads(uint32_t i) const;   // workgroup shape of the member's fused launches
+    // ||A x - b||_2 of member i (FP64; on its engine's fused kernels when it has one).  Synchronises.
+    double residualNorm(uint32_t i);
     // One eager iteration: total milliseconds of the fused launches and their count.
     double timeIteration(uint32_t* launches);
 
@@ -43,6 +45,7 @@ private:
     std::vector<Slot> slots_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
+    DeviceBuffer norm_scratch_;     // FP64 partial sums of the residual norm
 };
 
 }  // namespace etn

@@ -273,6 +273,13 @@ void BlockIterativeReconstructionOperator::doBackProjectionForBlock(uint32_t blo
 
 double BlockIterativeReconstructionOperator::residualNorm()
 {
+    if (fused_) {
+        // the fused kernels' forward-only passes (one volume sweep per group
+        // of projections instead of one generic launch per projection)
+        const double norm = fused_->residualNorm(0, norm_scratch_.as<double>(), stream_);
+        fused_->check(stream_);
+        return norm;
+    }
     double* partials = norm_scratch_.as<double>();
     double* total = partials + dev::kSsdBlocks;
     ETN_HIP(hipMemsetAsync(total, 0, sizeof(double), stream_));

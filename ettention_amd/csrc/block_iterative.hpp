@@ -65,6 +65,8 @@ public:
     uint32_t getPhysicalProjectionIndex(uint32_t block, uint32_t p) const;
     const ParallelScannerGeometry& geometry(uint32_t physical) const { return geometries_.at(physical); }
     const float* realProjection(uint32_t physical) const { return projections_ + (size_t)physical * pixels_; }
+    // Physical indices of the projections the residual norm runs over (all but the skip list).
+    const std::vector<uint32_t>& measuredProjections() const { return measured_; }
 
 private:
     void handleOneIteration();

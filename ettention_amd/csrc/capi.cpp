@@ -468,7 +468,7 @@ etn_status etn_batch_residual_norm(etn_batch* b, uint32_t member, double* out)
     return guarded([&] {
         need(b, "batch");
         need(out, "output");
-        *out = b->batch->member(member).residualNorm();
+        *out = b->batch->residualNorm(member);
     });
 }
 

@@ -71,10 +71,11 @@ __host__ __device__ inline int shape_band_floats(int threads, bool mask)
 struct FusedFp {
     float src[3], h[3], v[3], ray[3];
     const float* measured;       // nv x nu
-    float* residual;             // nv x nu, written
+    float* residual;             // nv x nu, written: measured - line integral (raw: the line integral)
     float* length;               // nv x nu, written
     float loc_c;
     int use_loc;
+    int raw;                     // write the virtual projection itself (residual norm passes)
 };
 
 struct FusedBp {
@@ -822,7 +823,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                 l = f.loc_c;
             }
             const size_t i = (size_t)w * nu + u;
-            global_ptr(f.residual)[i] = global_ptr(f.measured)[i] - pv;
+            global_ptr(f.residual)[i] = f.raw ? pv : global_ptr(f.measured)[i] - pv;
             global_ptr(f.length)[i] = l;
         }
     }
@@ -841,7 +842,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                         l = f.loc_c;
                     }
                     const size_t i = (size_t)w * nu + u;
-                    global_ptr(f.residual)[i] = global_ptr(f.measured)[i] - pv;
+                    global_ptr(f.residual)[i] = f.raw ? pv : global_ptr(f.measured)[i] - pv;
                     global_ptr(f.length)[i] = l;
                 }
             }
@@ -1292,6 +1293,7 @@ public:
     void setThreads(int threads, hipStream_t stream) override;
     int threads() const override { return threads_; }
     size_t memberCount() const override { return members_.size(); }
+    double residualNorm(size_t member, double* scratch, hipStream_t stream) override;
 
 private:
     struct Launch {
@@ -1481,6 +1483,82 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
 }
 
 void FusedEngineImpl::enqueueIteration(hipStream_t stream) { enqueue(sched_, 0, sched_.launches.size(), stream, nullptr); }
+
+double FusedEngineImpl::residualNorm(size_t member, double* scratch, hipStream_t stream)
+{
+    if (member >= members_.size()) fail("member index out of range");
+    const MemberPlan& m = members_[member];
+    BlockIterativeReconstructionOperator& op = *m.op;
+    const std::vector<uint32_t>& measured = op.measuredProjections();
+    const size_t M = members_.size();
+    const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, threads_ * dev::kMaxRaysPerThread /
+                                                                  ((m.tile.planes * m.rows_per_plane * m.nu)
+                                                                   << m.fp_shift)));
+    // virtual projection and ray length images for the projections of one chunk
+    DeviceBuffer images(sizeof(float) * m.pixels * 2 * (size_t)max_fp);
+    auto slot = [&](int k, int which) { return images.as<float>() + m.pixels * (2 * (size_t)k + which); };
+    // chunks of max_fp measured projections: forward-only passes by z direction
+    std::vector<dev::FusedPass> passes;
+    std::vector<size_t> chunk_end;     // passes of chunk c end at chunk_end[c]
+    for (size_t c0 = 0; c0 < measured.size(); c0 += (size_t)max_fp) {
+        dev::FusedPass neg{}, pos{};
+        neg.zdir = -1;
+        pos.zdir = 1;
+        for (size_t k = c0; k < std::min(measured.size(), c0 + (size_t)max_fp); ++k) {
+            const ParallelScannerGeometry& g = op.geometry(measured[k]);
+            dev::FusedFp f{};
+            g.source_base().store(f.src);
+            g.horizontal_pitch().store(f.h);
+            g.vertical_pitch().store(f.v);
+            g.ray_direction().store(f.ray);
+            f.measured = op.realProjection(measured[k]);
+            f.residual = slot((int)(k - c0), 0);
+            f.length = slot((int)(k - c0), 1);
+            f.raw = 1;
+            dev::FusedPass& p = f.ray[2] > 0.0f ? pos : neg;
+            p.fp[p.nfp++] = f;
+        }
+        if (neg.nfp) passes.push_back(neg);
+        if (pos.nfp) passes.push_back(pos);
+        chunk_end.push_back(passes.size());
+    }
+    std::vector<dev::FusedWork> work;
+    for (int j = 0; j < m.ny; j += m.tile.planes) work.push_back({(int)member, j});
+    DeviceBuffer dpasses(sizeof(dev::FusedPass) * std::max<size_t>(passes.size(), 1));
+    DeviceBuffer dwork(sizeof(dev::FusedWork) * work.size());
+    DeviceBuffer dtables(sizeof(const dev::FusedPass*) * std::max<size_t>(passes.size() * M, 1));
+    std::vector<const dev::FusedPass*> tables(passes.size() * M, nullptr);
+    for (size_t k = 0; k < passes.size(); ++k) tables[k * M + member] = dpasses.as<dev::FusedPass>() + k;
+    ETN_HIP(hipMemcpyAsync(dpasses.get(), passes.data(), sizeof(dev::FusedPass) * passes.size(),
+                           hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipMemcpyAsync(dwork.get(), work.data(), dwork.bytes(), hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipMemcpyAsync(dtables.get(), tables.data(), sizeof(const dev::FusedPass*) * tables.size(),
+                           hipMemcpyHostToDevice, stream));
+    double* partials = scratch;
+    double* total = scratch + dev::kSsdBlocks;
+    ETN_HIP(hipMemsetAsync(total, 0, sizeof(double), stream));
+    const size_t lds = sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx, m.row_floats,
+                                                                      mask_, threads_, m.rowsets);
+    size_t pass = 0;
+    for (size_t c = 0; c < chunk_end.size(); ++c) {
+        for (; pass < chunk_end[c]; ++pass) {
+            const int rays = (passes[pass].nfp * m.tile.planes * m.rows_per_plane * m.nu) << m.fp_shift;
+            const dev::FusedLaunch fl{threads_, select_rpt(rays, threads_), (int)work.size(),
+                                      vols_.as<dev::FusedVolume>(), dwork.as<dev::FusedWork>(),
+                                      dtables.as<const dev::FusedPass*>() + pass * M, lds};
+            dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, fl, stream);
+        }
+        // squared differences in projection order, as the unfused norm adds them
+        const size_t c0 = c * (size_t)max_fp;
+        for (size_t k = c0; k < std::min(measured.size(), c0 + (size_t)max_fp); ++k)
+            dev::launch_sum_squared_difference(op.realProjection(measured[k]), slot((int)(k - c0), 0), m.pixels,
+                                               partials, total, stream);
+    }
+    double host = 0.0;
+    ETN_HIP(hipMemcpyAsync(&host, total, sizeof(double), hipMemcpyDeviceToHost, stream));
+    ETN_HIP(hipStreamSynchronize(stream));
+    return std::sqrt(host);
+}
 
 void FusedEngineImpl::check(hipStream_t stream)
 {

@@ -59,6 +59,11 @@ public:
     virtual void setThreads(int threads, hipStream_t stream) = 0;
     virtual int threads() const = 0;
     virtual size_t memberCount() const = 0;
+    // ||A x - b||_2 of a member over its measured projections, FP64: the
+    // forward projections run as forward-only passes of the fused kernels,
+    // the squared differences are summed per projection in the member's
+    // projection order.  `scratch` holds kSsdBlocks + 1 doubles.  Synchronises.
+    virtual double residualNorm(size_t member, double* scratch, hipStream_t stream) = 0;
 };
 
 }  // namespace etn

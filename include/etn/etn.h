@@ -219,7 +219,9 @@ etn_status etn_reconstruction_run_iterations(etn_reconstruction* r, uint32_t ite
 etn_status etn_reconstruction_volume(etn_reconstruction* r, void** device_volume, size_t* bytes);
 /* Replace the working volume (volume.initFile): device source of the same type and size. */
 etn_status etn_reconstruction_set_volume(etn_reconstruction* r, const void* device_volume);
-/* ||A x - b||_2 over every used projection, FP64 accumulation; synchronises. */
+/* ||A x - b||_2 over every used projection, FP64 accumulation; synchronises.
+ * On the fused path the forward projections run as forward-only fused passes
+ * (bit-identical to the generic kernels' norm with exact arithmetic). */
 etn_status etn_reconstruction_residual_norm(etn_reconstruction* r, double* out);
 etn_status etn_reconstruction_synchronize(etn_reconstruction* r);
 /* 1 when the fused plane kernels run this reconstruction, 0 for the generic kernels. */
