@@ -1035,6 +1035,35 @@ VolumeProperties whole_volume(const VolumeProperties& vp)
     return VolumeProperties(g);
 }
 
+// The fast lookup skips the column clamp: stage every column u - 0.5 can
+// floor to for a voxel centre (and every sample of the BP pattern) under any
+// of the geometries, plus one column either side.  False when the span is
+// unbounded.
+bool fast_row_span(const VolumeProperties& vp, const std::vector<const ParallelScannerGeometry*>& geoms, int nu,
+                   int bp_s, int& pad_l, int& row_len)
+{
+    const Vec3f vs = vp.voxel_size(), lo = vp.bbox_min(), hi = vp.bbox_max();
+    double umin = 1e300, umax = -1e300;
+    for (const ParallelScannerGeometry* g : geoms) {
+        const Matrix4x4 pm = g->projection_matrix();
+        const float* mv = pm.data();
+        for (const VoxelSample& smp : bp_sample_pattern(*g, vs, (uint32_t)bp_s))
+            for (int c = 0; c < 8; ++c) {
+                const double px = ((c & 1) ? hi.x - 0.5 * vs.x : lo.x + 0.5 * vs.x) + smp.dx;
+                const double py = ((c & 2) ? hi.y - 0.5 * vs.y : lo.y + 0.5 * vs.y) + smp.dy;
+                const double pz = ((c & 4) ? hi.z - 0.5 * vs.z : lo.z + 0.5 * vs.z) + smp.dz;
+                const double u = mv[0] * px + mv[4] * py + mv[8] * pz + mv[12];
+                umin = std::min(umin, u);
+                umax = std::max(umax, u);
+            }
+    }
+    if (!(umin > -1e6 && umax < 1e6)) return false;
+    const int bu_lo = (int)std::floor(umin - 0.5) - 1, bu_hi = (int)std::floor(umax - 0.5) + 1;
+    pad_l = std::max(dev::kRowPad, -bu_lo);
+    row_len = pad_l + std::max(nu + dev::kRowPad, bu_hi + 1);
+    return true;
+}
+
 // Eligibility of one reconstruction and its plane/row tables; false when it
 // has no fused form.
 bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruction_desc& d, bool exact,
@@ -1147,29 +1176,11 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     out.pad_l = dev::kRowPad;
     out.row_len = nu + 2 * dev::kRowPad;
     if (!exact) {
-        // the fast lookup skips the column clamp: stage every column u - 0.5 can
-        // floor to for a voxel centre of this member, plus one column either side
-        const Vec3f vs = vp.voxel_size(), lo = vp.bbox_min(), hi = vp.bbox_max();
-        double umin = 1e300, umax = -1e300;
+        std::vector<const ParallelScannerGeometry*> geoms;
         for (uint32_t b = 0; b < op.GetBlockCount(); ++b)
-            for (uint32_t p = 0; p < op.getBlockSize(b); ++p) {
-                const ParallelScannerGeometry& g = op.geometry(op.getPhysicalProjectionIndex(b, p));
-                const Matrix4x4 pm = g.projection_matrix();
-                const float* mv = pm.data();
-                for (const VoxelSample& smp : bp_sample_pattern(g, vs, (uint32_t)bp_s))
-                    for (int c = 0; c < 8; ++c) {
-                        const double px = ((c & 1) ? hi.x - 0.5 * vs.x : lo.x + 0.5 * vs.x) + smp.dx;
-                        const double py = ((c & 2) ? hi.y - 0.5 * vs.y : lo.y + 0.5 * vs.y) + smp.dy;
-                        const double pz = ((c & 4) ? hi.z - 0.5 * vs.z : lo.z + 0.5 * vs.z) + smp.dz;
-                        const double u = mv[0] * px + mv[4] * py + mv[8] * pz + mv[12];
-                        umin = std::min(umin, u);
-                        umax = std::max(umax, u);
-                    }
-            }
-        if (!(umin > -1e6 && umax < 1e6)) return false;
-        const int bu_lo = (int)std::floor(umin - 0.5) - 1, bu_hi = (int)std::floor(umax - 0.5) + 1;
-        out.pad_l = std::max(dev::kRowPad, -bu_lo);
-        out.row_len = out.pad_l + std::max(nu + dev::kRowPad, bu_hi + 1);
+            for (uint32_t p = 0; p < op.getBlockSize(b); ++p)
+                geoms.push_back(&op.geometry(op.getPhysicalProjectionIndex(b, p)));
+        if (!fast_row_span(vp, geoms, nu, bp_s, out.pad_l, out.row_len)) return false;
     }
     out.row_floats = out.row_len * (exact ? 8 : 4);
     out.mask = d.mask;
@@ -1740,6 +1751,144 @@ void FusedEngineImpl::autotune(hipStream_t stream)
 }
 
 }  // namespace
+
+namespace {
+
+class FusedBackProjectorImpl final : public FusedBackProjector {
+public:
+    FusedBackProjectorImpl(GpuVolume* volume, const std::vector<ParallelScannerGeometry>& geometries, MemberPlan plan)
+        : volume_(volume), geometries_(geometries), m_(std::move(plan))
+    {
+        diag_.allocate(sizeof(int));
+        ETN_HIP(hipMemset(diag_.get(), 0, sizeof(int)));
+    }
+    void backProject(const std::vector<uint32_t>& indices, const std::vector<const float*>& residual,
+                     const std::vector<const float*>& length, float lambda, hipStream_t stream,
+                     std::vector<float>* launch_ms) override;
+
+    GpuVolume* volume_;
+    std::vector<ParallelScannerGeometry> geometries_;
+    MemberPlan m_;
+    DeviceBuffer diag_;
+};
+
+void FusedBackProjectorImpl::backProject(const std::vector<uint32_t>& indices,
+                                         const std::vector<const float*>& residual,
+                                         const std::vector<const float*>& length, float lambda, hipStream_t stream,
+                                         std::vector<float>* launch_ms)
+{
+    if (residual.size() != indices.size() || length.size() != indices.size())
+        fail("fused back projector: one residual and one length image per projection");
+    const VolumeProperties vp = whole_volume(volume_->properties());
+    std::vector<dev::FusedPass> passes;
+    for (size_t k = 0; k < indices.size(); ++k) {
+        if (indices[k] >= geometries_.size()) fail("fused back projector: projection index out of range");
+        if (k % dev::kMaxFusedBp == 0) {
+            passes.push_back(dev::FusedPass{});
+            passes.back().zdir = 1;
+        }
+        const ParallelScannerGeometry& g = geometries_[indices[k]];
+        dev::FusedBp e{};
+        std::memcpy(e.m, g.projection_matrix().data(), sizeof(e.m));
+        e.lambda = lambda;
+        e.residual = residual[k];
+        e.length = length[k];
+        const std::vector<VoxelSample> smp = bp_sample_pattern(g, vp.voxel_size(), (uint32_t)m_.bp_samples);
+        e.sample = make_float4(smp[0].dx, smp[0].dy, smp[0].dz, smp[0].length);
+        e.nsamples = (int)smp.size();
+        e.per_dir = m_.rowsets;
+        for (size_t t = 0; t < smp.size(); ++t) e.samples[t] = make_float4(smp[t].dx, smp[t].dy, smp[t].dz, smp[t].length);
+        dev::FusedPass& p = passes.back();
+        p.bp[p.nbp++] = e;
+    }
+    if (passes.empty()) return;
+    dev::FusedVolume fv{};
+    fv.data = volume_->data();
+    fv.slab = make_slab_params(vp, 0);
+    fv.nx = m_.nx;
+    fv.ny = m_.ny;
+    fv.nz = m_.nz;
+    fv.nu = m_.nu;
+    fv.nv = m_.nv;
+    fv.planes = m_.tile.planes;
+    fv.band = m_.tile.band;
+    fv.rows_per_plane = m_.rows_per_plane;
+    fv.row_len = m_.row_len;
+    fv.pad_l = m_.pad_l;
+    fv.fp_n = 1;
+    fv.rowsets = m_.rowsets;
+    fv.diag = diag_.as<int>();
+    std::vector<dev::FusedWork> work;
+    for (int j = 0; j < m_.ny; j += m_.tile.planes) work.push_back({0, j});
+    DeviceBuffer dvol(sizeof(dev::FusedVolume)), dpasses(sizeof(dev::FusedPass) * passes.size());
+    DeviceBuffer dwork(sizeof(dev::FusedWork) * work.size()), dtables(sizeof(const dev::FusedPass*) * passes.size());
+    std::vector<const dev::FusedPass*> tables(passes.size());
+    for (size_t k = 0; k < passes.size(); ++k) tables[k] = dpasses.as<dev::FusedPass>() + k;
+    ETN_HIP(hipMemcpyAsync(dvol.get(), &fv, sizeof(fv), hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipMemcpyAsync(dpasses.get(), passes.data(), dpasses.bytes(), hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipMemcpyAsync(dwork.get(), work.data(), dwork.bytes(), hipMemcpyHostToDevice, stream));
+    ETN_HIP(hipMemcpyAsync(dtables.get(), tables.data(), dtables.bytes(), hipMemcpyHostToDevice, stream));
+    const size_t lds = sizeof(float) * (size_t)dev::fused_lds_floats(m_.tile.planes, m_.tile.band, m_.nx,
+                                                                      m_.row_floats, false, m_.threads, m_.rowsets);
+    std::vector<hipEvent_t> ev(launch_ms ? 2 * passes.size() : 0);
+    for (auto& e : ev) ETN_HIP(hipEventCreate(&e));
+    for (size_t k = 0; k < passes.size(); ++k) {
+        const dev::FusedLaunch fl{m_.threads, 1, (int)work.size(), dvol.as<dev::FusedVolume>(),
+                                  dwork.as<dev::FusedWork>(), dtables.as<const dev::FusedPass*>() + k, lds};
+        if (launch_ms) ETN_HIP(hipEventRecord(ev[2 * k], stream));
+        dev::launch_fused(volume_->voxel_type(), true, false, false, false, m_.oversampled(), fl, stream);
+        if (launch_ms) ETN_HIP(hipEventRecord(ev[2 * k + 1], stream));
+    }
+    ETN_HIP(hipStreamSynchronize(stream));   // the tables above are freed on return
+    for (size_t k = 0; launch_ms && k < passes.size(); ++k) {
+        float ms = 0.0f;
+        ETN_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+        if (passes[k].nbp == dev::kMaxFusedBp) launch_ms->push_back(ms);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+}  // namespace
+
+std::unique_ptr<FusedBackProjector> FusedBackProjector::create(GpuVolume* volume,
+                                                               const std::vector<ParallelScannerGeometry>& geometries,
+                                                               uint32_t nu, uint32_t nv, uint32_t bp_samples,
+                                                               hipStream_t)
+{
+    const VolumeProperties vp = whole_volume(volume->properties());
+    MemberPlan m;
+    m.nx = (int)vp.nx();
+    m.ny = (int)vp.ny();
+    m.nz = (int)vp.nz();
+    m.nu = (int)nu;
+    m.nv = (int)nv;
+    if (m.nx % 4 || geometries.empty()) return nullptr;
+    const int bp_s = (int)(bp_samples ? bp_samples : 1);
+    const int pd = (int)std::lround(std::sqrt((double)bp_s));
+    if (bp_s > dev::kMaxFusedBpSamples || pd * pd != bp_s) return nullptr;
+    std::vector<const ParallelScannerGeometry*> geoms;
+    for (const ParallelScannerGeometry& g : geometries) {
+        // the v coordinate of a voxel depends on its y only, w == 1
+        if (g.ray_direction().y != 0.0f || g.horizontal_pitch().y != 0.0f) return nullptr;
+        const Matrix4x4 pm = g.projection_matrix();
+        const float* mv = pm.data();
+        if (mv[1] != 0.0f || mv[9] != 0.0f || mv[3] != 0.0f || mv[7] != 0.0f || mv[11] != 0.0f || mv[15] != 1.0f)
+            return nullptr;
+        geoms.push_back(&g);
+    }
+    if (!fast_row_span(vp, geoms, m.nu, bp_s, m.pad_l, m.row_len)) return nullptr;
+    m.pixels = (size_t)nu * nv;
+    m.rows_per_plane = 1;
+    m.row_floats = m.row_len * 4;
+    m.threads = dev::kMaxThreads;
+    m.bp_samples = bp_s;
+    m.rowsets = pd;
+    if (dev::fused_lds_floats(1, 1, m.nx, m.row_floats, false, m.threads, m.rowsets) >
+        dev::shape_lds_floats(m.threads) || max_band(m, 1) < 1)
+        return nullptr;
+    m.tile = default_tile(m);
+    return std::make_unique<FusedBackProjectorImpl>(volume, geometries, std::move(m));
+}
 
 std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Member>& members, bool exact,
                                                            hipStream_t stream, std::vector<bool>* taken)

@@ -16,6 +16,31 @@
 namespace etn {
 
 class BlockIterativeReconstructionOperator;
+class GpuVolume;
+class ParallelScannerGeometry;
+
+// Back projections of many projections into one volume with the fused
+// kernels' back-projection phase alone (fast arithmetic): each launch applies
+// up to eight projections in one sweep of the volume, in list order, rounding
+// to the storage type after each as separate launches would.  For callers of
+// the BP other than the block-iterative loop (weighted back projection).
+class FusedBackProjector {
+public:
+    // Null when the geometries have no fused form (x-axis tilt, nx not a
+    // multiple of 4, a sample count the fused kernels do not take).
+    static std::unique_ptr<FusedBackProjector> create(GpuVolume* volume,
+                                                      const std::vector<ParallelScannerGeometry>& geometries,
+                                                      uint32_t nu, uint32_t nv, uint32_t bp_samples, hipStream_t stream);
+    virtual ~FusedBackProjector() = default;
+    // Back-projects projections indices[k] (into the geometry list) with
+    // residual[k] and ray length[k] (device nv x nu images) and weight
+    // lambda (already divided by the block size).  Synchronises.  With
+    // `launch_ms`, appends the event-timed duration of every launch that
+    // carries eight projections.
+    virtual void backProject(const std::vector<uint32_t>& indices, const std::vector<const float*>& residual,
+                             const std::vector<const float*>& length, float lambda, hipStream_t stream,
+                             std::vector<float>* launch_ms = nullptr) = 0;
+};
 
 class FusedPlaneEngine {
 public:

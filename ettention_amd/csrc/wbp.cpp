@@ -94,6 +94,9 @@ WeightedBackProjection::WeightedBackProjection(const etn_reconstruction_desc& d,
     taps_.allocate(k.size() * sizeof(float));
     ETN_HIP(hipMemcpyAsync(taps_.get(), k.data(), taps_.bytes(), hipMemcpyHostToDevice, stream_));
     ETN_HIP(hipStreamSynchronize(stream_));
+    if ((d.flags & (ETN_RECON_FLAG_GENERIC_KERNELS | ETN_RECON_FLAG_EXACT_ARITHMETIC)) == 0)
+        fused_ = FusedBackProjector::create(volume_.get(), geometries_, nu, nv, d.bp_samples ? d.bp_samples : 1,
+                                            stream_);
 }
 
 WeightedBackProjection::~WeightedBackProjection()
@@ -112,6 +115,30 @@ void WeightedBackProjection::run() { runNiterations(count_); }
 void WeightedBackProjection::runNiterations(uint32_t n)
 {
     if (n > count_) fail("WBP: more projections requested than the stack holds");
+    if (fused_) {
+        // filtered images and ray lengths of up to 64 projections, then their
+        // back projections in stack order, eight per volume sweep
+        constexpr uint32_t kChunk = 64;
+        DeviceBuffer chunk(sizeof(float) * 2 * pixels_ * std::min(n, kChunk));
+        float* const saved = length_.data;
+        for (uint32_t i0 = 0; i0 < n; i0 += kChunk) {
+            std::vector<uint32_t> idx;
+            std::vector<const float*> res, len;
+            for (uint32_t i = i0; i < std::min(n, i0 + kChunk); ++i) {
+                float* f = chunk.as<float>() + 2 * pixels_ * (i - i0);
+                setup_->setScannerGeometry(geometries_[i]);
+                filter(projections_ + (size_t)i * pixels_, f);
+                length_.data = f + pixels_;
+                ray_length_->run();
+                idx.push_back(i);
+                res.push_back(f);
+                len.push_back(f + pixels_);
+            }
+            length_.data = saved;
+            fused_->backProject(idx, res, len, lambda_ / (float)count_, stream_, launch_ms_);
+        }
+        return;
+    }
     for (uint32_t i = 0; i < n; ++i) {
         setup_->setScannerGeometry(geometries_[i]);
         filter(projections_ + (size_t)i * pixels_, filtered_.data);
@@ -123,5 +150,46 @@ void WeightedBackProjection::runNiterations(uint32_t n)
 }
 
 void WeightedBackProjection::synchronize() { ETN_HIP(hipStreamSynchronize(stream_)); }
+
+double WeightedBackProjection::timeKernels(int* kernel, uint32_t* launches)
+{
+    std::vector<float> ms;
+    if (fused_) {
+        *kernel = 0;
+        launch_ms_ = &ms;
+        try {
+            run();
+        } catch (...) {
+            launch_ms_ = nullptr;
+            throw;
+        }
+        launch_ms_ = nullptr;
+    } else {
+        *kernel = 2;
+        hipEvent_t a, b;
+        ETN_HIP(hipEventCreate(&a));
+        ETN_HIP(hipEventCreate(&b));
+        for (uint32_t i = 0; i < count_; ++i) {
+            setup_->setScannerGeometry(geometries_[i]);
+            filter(projections_ + (size_t)i * pixels_, filtered_.data);
+            ray_length_->run();
+            backward_->setInput(&filtered_, &length_, count_);
+            ETN_HIP(hipEventRecord(a, stream_));
+            backward_->execute();
+            ETN_HIP(hipEventRecord(b, stream_));
+            backward_->setInput(nullptr, nullptr, 1);
+            ETN_HIP(hipEventSynchronize(b));
+            float t = 0.0f;
+            ETN_HIP(hipEventElapsedTime(&t, a, b));
+            ms.push_back(t);
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    *launches = (uint32_t)ms.size();
+    double total = 0.0;
+    for (float t : ms) total += t;
+    return ms.empty() ? 0.0 : total / ms.size();
+}
 
 }  // namespace etn

@@ -8,6 +8,7 @@
 #include <memory>
 #include <vector>
 
+#include "fused.hpp"
 #include "kernels.hpp"
 #include "operators.hpp"
 #include "reconstruction.hpp"
@@ -29,6 +30,14 @@ public:
     GpuVolume* getReconstructedVolume() override { return volume_.get(); }
     void synchronize() override;
     hipStream_t stream() const override { return stream_; }
+    // Single-axis tilt geometries back-project on the fused kernels, eight
+    // projections per volume sweep (fast arithmetic), unless the generic
+    // kernels or exact arithmetic are asked for.
+    bool usesFusedPath() const override { return fused_ != nullptr; }
+    // One more run() with HIP events around the back projections: the average
+    // fused launch carrying eight projections (*kernel = 0), or the generic
+    // back projection (*kernel = 2).  Adds to the volume.
+    double timeKernels(int* kernel, uint32_t* launches) override;
 
     // GenerateRamLakKernel.cl: value at position p = i - resolution / 2 is 1/4
     // at 0, 0 at even p, -1 / (p^2 pi^2) at odd p, in float.
@@ -56,6 +65,8 @@ private:
     std::unique_ptr<ComputeRayLengthKernel> ray_length_;
     DeviceBuffer taps_;
     dev::RampFilterParams params_;
+    std::unique_ptr<FusedBackProjector> fused_;
+    std::vector<float>* launch_ms_ = nullptr;    // set while timeKernels runs
 };
 
 }  // namespace etn
