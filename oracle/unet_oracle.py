@@ -2,8 +2,8 @@
 correction network (3-D U-Net, 3 levels, 16/32/64 channels, 3x3x3 convs).
 Its input is the pair (x, d): an early SART iterate and its last update.
 
-The network has no counterpart in the reference: it is the north-star
-extension that predicts a late SART iterate from an early one.  This module is
+The network has no counterpart in the reference: it is an extension that
+predicts a late SART iterate from an early one.  This module is
 the mathematical definition the HIP kernels are checked against (forward
 activations and every parameter gradient); its own gradients are pinned by
 central finite differences in tests/test_unet_oracle.py.  Only tests/ may

@@ -2,8 +2,8 @@
 
 FP32 tolerance: the kernels follow the oracle's float operations term by term
 (no contraction), so forward and back projections are expected to be
-bit-identical; the asserted bound is relative L2 <= 1e-6 (the north-star
-parity bound is 1e-5)."""
+bit-identical; the asserted bound is relative L2 <= 1e-6 (the FP32 parity
+bound for whole reconstructions is 1e-5)."""
 import numpy as np
 import pytest
 

@@ -1,6 +1,6 @@
 """The block-iterative reconstruction on the GPU (etn_reconstruction_*) against
 the oracle's loop: per volume after N iterations, relative L2 <= 1e-5 in FP32
-(the north-star parity bound), and the reference's own convergence bounds."""
+(the FP32 parity bound), and the reference's own convergence bounds."""
 import numpy as np
 import pytest
 
