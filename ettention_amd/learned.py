@@ -12,7 +12,7 @@ with its half-pixel offset, is not the forward projector's adjoint), on
 noiseless data as well.  The late iterate is therefore the one with the
 lowest residual within k_max iterations, not iterate k_max itself.
 
-Plain SART stays the judge: a jump is accepted only if its projection
+Plain SART stays the arbiter: a jump is accepted only if its projection
 residual is below what plain SART reaches in the same wall-clock time.
 """
 from __future__ import annotations
@@ -293,7 +293,7 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
 
 
 def learned_sart(case, net, scale, k0=3, lam=0.3, order="random", voxel_type="bfloat16"):
-    """SART with the learned jump from iterate k0, plain SART as the judge:
+    """SART with the learned jump from iterate k0, plain SART as the arbiter:
     returns (volume, accepted, jump residual, plain residual), where plain SART
     gets the iterations that fit in the jump's wall-clock time and its volume
     is returned whenever the jump does not beat it."""

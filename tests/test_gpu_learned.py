@@ -2,7 +2,7 @@
 
 The network is trained by the library's own kernels on samples of seeded 64^3
 phantoms -- bf16 SART iterate 3 and its last update as input, the FP32 SART
-minimum-residual iterate as target -- and applied to a held-out phantom.  Plain SART is the judge: the jump must lower the projection
+minimum-residual iterate as target -- and applied to a held-out phantom.  Plain SART is the arbiter: the jump must lower the projection
 residual ||Ax - b|| (FP64, the reconstruction's own norm) below that of the
 iterate it started from, and the acceptance rule must keep plain SART's
 result whenever the jump loses to the iterations plain SART runs in the same
