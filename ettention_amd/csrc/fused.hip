@@ -910,16 +910,21 @@ void launch_fused_fast(etn_voxel_type vt, bool vec, bool cap, const FusedLaunch&
 
 // vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049;
 // exact: the generic kernels' operations and rounding order; mask: the launch
-// carries prior-knowledge masks; over: oversampled FP and BP (mask and over
-// with fast arithmetic only)
+// carries prior-knowledge masks; over: oversampled FP and BP (fast arithmetic only)
 void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask, bool over, const FusedLaunch& l,
                   hipStream_t stream)
 {
     if (exact) {
-        if (mask || over) unsupported("fused plane kernel: a mask or oversampling needs fast arithmetic");
-        if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, false, false>(vec, cap, l, stream);
-        else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, false, false>(vec, cap, l, stream);
-        else launch_fused_typed<BF16, false, false, false>(vec, cap, l, stream);
+        if (over) unsupported("fused plane kernel: oversampling needs fast arithmetic");
+        if (mask) {
+            if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, true, false>(vec, cap, l, stream);
+            else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, true, false>(vec, cap, l, stream);
+            else launch_fused_typed<BF16, false, true, false>(vec, cap, l, stream);
+        } else {
+            if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, false, false>(vec, cap, l, stream);
+            else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, false, false>(vec, cap, l, stream);
+            else launch_fused_typed<BF16, false, false, false>(vec, cap, l, stream);
+        }
     } else if (over) {
         if (mask) launch_fused_fast<true, true>(vt, vec, cap, l, stream);
         else launch_fused_fast<false, true>(vt, vec, cap, l, stream);
@@ -1086,7 +1091,6 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     if (fp_n != 1 && op.getReconstructedVolume()->properties().subvolume_count() != 1) return false;
     int fp_shift = 0;
     while ((1 << fp_shift) < fp_n) ++fp_shift;     // ray slots per pixel: sub-ray columns si
-    if (d.mask && exact) return false;   // the exact kernels carry no mask: generic kernels
     if (reinterpret_cast<uintptr_t>(d.mask) % 4) return false;   // masks are read 4 bytes at a time
     const int nx = (int)vp.nx(), ny = (int)vp.ny(), nz = (int)vp.nz();
     const int nu = (int)d.base_geometry.resolution[0], nv = (int)d.base_geometry.resolution[1];

@@ -115,8 +115,9 @@ def test_ineligible_configurations_use_generic_kernels(synthetic):
                dict(fp_samples=2, volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=4)),
                dict(volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=2), flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
                dict(x_axis_tilt=5.0),
-               dict(mask=torch.full((64, 64, 64), 255, dtype=torch.uint8, device="cuda"),
-                    flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC)):
+               # a mask one byte off 4-byte alignment (the fused kernels read 4 bytes at a time)
+               dict(mask=torch.full((64 * 64 * 64 + 4,), 255, dtype=torch.uint8, device="cuda")[1:1 + 64 ** 3]
+                    .view(64, 64, 64))):
         args = dict(volume=E.VolumeGeometry.make((64, 64, 64)), base_geometry=TEST_BASE, tilt_angles=tilts[sub],
                     projections=p, algorithm="sart")
         args.update(kw)
@@ -388,3 +389,46 @@ def test_residual_norm_on_fused_kernels(synthetic, kw):
             assert abs(r.residual_norm() - n_generic) <= 1e-6 * n_generic
         elif fused:
             assert r.residual_norm() == n_generic
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(64, 64, 64), geometry="test", algorithm="blockiterative", block_size=5, lam=0.8,
+         projection_iteration="maxangle", use_long_object_compensation=True),
+    dict(shape=(66, 48, 40), geometry="mrc", algorithm="sart", lam=0.3, projection_iteration="random"),
+    dict(shape=(64, 64, 64), geometry="test", algorithm="sart", lam=0.3, projection_iteration="identity",
+         voxel_type="half"),
+])
+def test_mask_exact_fused_equals_generic(synthetic, case):
+    """With exact arithmetic the masked fused kernels perform the generic
+    kernels' operations (dt * weight summed into the ray length, weight * sum
+    added to the voxel), so volumes and residual norms are bit-identical."""
+    case = dict(case)
+    shape, geometry = case.pop("shape"), case.pop("geometry")
+    nx, ny, nz = shape
+    vg = E.VolumeGeometry.make(shape)
+    if geometry == "test":
+        _, tilts, projs = synthetic
+        tilts, projs = tilts[::5], np.ascontiguousarray(projs[::5])
+        base = TEST_BASE
+    else:
+        base = E.ScannerGeometry.mrc_base(nx, ny)
+        tilts = S.tilt_series(60.0, 5.0)
+        truth_t = dev(S.ellipsoid_phantom((nz, ny, nx), seed=80))
+        pt = torch.zeros((len(tilts), ny, nx), dtype=torch.float32, device="cuda")
+        scratch = torch.zeros((ny, nx), dtype=torch.float32, device="cuda")
+        for i, t in enumerate(tilts):
+            scratch.zero_()
+            A.forward_project(vg, truth_t, A.rotate(base, float(t)), pt[i], scratch)
+        projs = pt.cpu().numpy()
+    mask_t = dev(seeded_mask(shape, 8))
+    out = []
+    for flags in (E.ETN_RECON_FLAG_EXACT_ARITHMETIC, E.ETN_RECON_FLAG_GENERIC_KERNELS):
+        r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                                   projections=dev(projs), iterations=2, flags=flags, mask=mask_t,
+                                                   **case)
+        assert r.uses_fused_path() == (flags == E.ETN_RECON_FLAG_EXACT_ARITHMETIC)
+        r.run()
+        r.synchronize()
+        out.append((r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy(), r.residual_norm()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
