@@ -327,7 +327,7 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // kMask applies the prior-knowledge mask (ImageHandler.hl:85-89): the ray
 // length sums dt * weight, the update is weight * sum.  A member without a
 // mask gets weight 1 (every update then equals the unmasked kernel's).
-// kOver (fast arithmetic) oversamples: fp_n x fp_n sub-rays per detector pixel
+// kOver oversamples: fp_n x fp_n sub-rays per detector pixel
 // (ForwardProjectionKernel.cpp:65-68), whose sums the pixel averages, and a
 // per_dir x per_dir sample pattern per voxel (ParallelBeamsBackProjectionKernel
 // .cpp:13-103), each pattern row with its own staged row pair.  Sub-rays (si,
@@ -680,12 +680,29 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                             float u = m0 * (cx + sm.x) + pt.x;
                             u = u + m8 * (cz + sm.z);
                             u = u + m12 * 1.0f;
-                            u = u - 0.5f;
-                            const float bu = floorf(u);
-                            const float tx = u - bu;
-                            const float4 c = recs[(p * rowsets + ys) * row_len + (int)bu + pad_l];
-                            const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
-                            const float contrib = Rv * (lam * sm.w) * __builtin_amdgcn_rcpf(L);
+                            float L, contrib;
+                            if (kFast) {
+                                u = u - 0.5f;
+                                const float bu = floorf(u);
+                                const float tx = u - bu;
+                                const float4 c = recs[(p * rowsets + ys) * row_len + (int)bu + pad_l];
+                                const float Rv = fmaf(c.y, tx, c.x);
+                                L = fmaf(c.w, tx, c.z);
+                                contrib = Rv * (lam * sm.w) * __builtin_amdgcn_rcpf(L);
+                            } else {
+                                // bp_voxel_sum's bilinear lookups and weight, operation for operation
+                                float bu;
+                                const float tx = cl_fract(u - 0.5f, &bu);
+                                const int i = padded_index((int)bu, nu);
+                                const float4* pr = recs + (p * rowsets + ys) * 2 * row_len;
+                                const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
+                                const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
+                                L = L0 + (L1 - L0) * pt.y;
+                                const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
+                                const float Rv = R0 + (R1 - R0) * pt.y;
+                                const float wgt = sm.w / L;
+                                contrib = Rv * lam * wgt;
+                            }
                             const bool ok = L > kEpsilon;
                             sum += ok ? contrib : 0.0f;
                             used += ok ? 1 : 0;
@@ -910,12 +927,21 @@ void launch_fused_fast(etn_voxel_type vt, bool vec, bool cap, const FusedLaunch&
 
 // vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049;
 // exact: the generic kernels' operations and rounding order; mask: the launch
-// carries prior-knowledge masks; over: oversampled FP and BP (fast arithmetic only)
+// carries prior-knowledge masks; over: oversampled FP and BP
 void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask, bool over, const FusedLaunch& l,
                   hipStream_t stream)
 {
-    if (exact) {
-        if (over) unsupported("fused plane kernel: oversampling needs fast arithmetic");
+    if (exact && over) {
+        if (mask) {
+            if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, true, true>(vec, cap, l, stream);
+            else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, true, true>(vec, cap, l, stream);
+            else launch_fused_typed<BF16, false, true, true>(vec, cap, l, stream);
+        } else {
+            if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, false, true>(vec, cap, l, stream);
+            else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, false, true>(vec, cap, l, stream);
+            else launch_fused_typed<BF16, false, false, true>(vec, cap, l, stream);
+        }
+    } else if (exact) {
         if (mask) {
             if (vt == ETN_VOXEL_FLOAT32) launch_fused_typed<F32, false, true, false>(vec, cap, l, stream);
             else if (vt == ETN_VOXEL_FLOAT16) launch_fused_typed<F16, false, true, false>(vec, cap, l, stream);
@@ -1077,13 +1103,11 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     if (exact && op.getReconstructedVolume()->properties().subvolume_count() != 1) return false;
     const VolumeProperties whole = whole_volume(op.getReconstructedVolume()->properties());
     const VolumeProperties& vp = whole;
-    // oversampling: up to 4 x 4 forward sub-rays and a 4 x 4 back projection
-    // pattern, with fast arithmetic (the exact kernels take one sample each)
+    // oversampling: up to 4 x 4 forward sub-rays and a 4 x 4 back projection pattern
     const int fp_n = (int)(d.fp_samples ? d.fp_samples : 1);
     const int bp_s = (int)(d.bp_samples ? d.bp_samples : 1);
     const int bp_pd = (int)std::lround(std::sqrt((double)bp_s));
     if (fp_n > dev::kMaxFusedFpPerAxis || bp_s > dev::kMaxFusedBpSamples || bp_pd * bp_pd != bp_s) return false;
-    if (exact && (fp_n != 1 || bp_s != 1)) return false;
     // forward sub-rays restarted at every slab face (the reference's slab-wise
     // FP) differ from one sweep by up to 2.4e-4 relative L2 after two SART
     // iterations (4 x 4 sub-rays, 2 or 4 slabs): beyond the FP32 bound, so an

@@ -15,11 +15,11 @@ from ettention_amd import build as B
 def test_no_packed_fp32_in_device_code(src):
     assert "-packed-fp32-ops" in B.FLAGS
     with tempfile.TemporaryDirectory() as tmp:
-        cmd = [B.hipcc()] + B.FLAGS + ["-c", os.path.join(B.CSRC, src), "-o", os.path.join(tmp, "k.o"),
-                                       "--save-temps"]
+        # the device pass alone, to gfx950 assembly
+        out = os.path.join(tmp, "k.s")
+        cmd = [B.hipcc()] + B.FLAGS + ["--cuda-device-only", "-S", os.path.join(B.CSRC, src), "-o", out]
         subprocess.run(cmd, cwd=tmp, check=True, capture_output=True)
-        asm = [f for f in os.listdir(tmp) if f.endswith(".s") and "gfx950" in f]
-        assert asm, os.listdir(tmp)
-        text = open(os.path.join(tmp, asm[0])).read()
+        text = open(out).read()
+    assert "amdgcn" in text and "gfx950" in text
     assert "_kernel" in text
     assert not re.search(r"v_pk_(add|mul|fma)_f32", text)

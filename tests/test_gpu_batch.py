@@ -55,7 +55,7 @@ def members():
         member((96, 64, 32), 33, 4003),
         member((30, 30, 30), 21, 4004),                          # nx % 4 != 0: its own engine
         member((40, 40, 24), 19, 4005, voxel_type="bfloat16"),  # another storage type: its own engine
-        member((32, 32, 32), 17, 4006, fp_samples=2),            # no fused form: generic kernels in the batch
+        member((32, 32, 32), 17, 4006, fp_samples=2),            # oversampled: its own engine
         member((64, 64, 64), 29, 4007, flags=E.ETN_RECON_FLAG_GENERIC_KERNELS),
     ]
 
@@ -66,7 +66,7 @@ EXACT = E.ETN_RECON_FLAG_EXACT_ARITHMETIC
 def test_batch_members_equal_standalone_generic(members):
     b = A.BatchReconstruction(members, flags=EXACT)
     fused = [b.uses_fused_path(i) for i in range(len(b))]
-    assert fused == [True, True, True, True, True, True, False, False]
+    assert fused == [True, True, True, True, True, True, True, False]
     b.run_iterations(2)
     b.synchronize()
     for i, m in enumerate(members):

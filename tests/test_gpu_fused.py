@@ -109,9 +109,9 @@ def test_ineligible_configurations_use_generic_kernels(synthetic):
     gt, tilts, projs = synthetic
     sub = slice(None, None, 20)
     p = dev(projs[sub])
-    for kw in (dict(fp_samples=4, flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
-               dict(bp_samples=4, flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
-               dict(fp_samples=5), dict(bp_samples=25),
+    for kw in (dict(fp_samples=5), dict(bp_samples=25),
+               dict(fp_samples=2, flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC,
+                    volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=2)),
                dict(fp_samples=2, volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=4)),
                dict(volume=E.VolumeGeometry.make((64, 64, 64), subvolumes=2), flags=E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
                dict(x_axis_tilt=5.0),
@@ -426,6 +426,35 @@ def test_mask_exact_fused_equals_generic(synthetic, case):
         r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
                                                    projections=dev(projs), iterations=2, flags=flags, mask=mask_t,
                                                    **case)
+        assert r.uses_fused_path() == (flags == E.ETN_RECON_FLAG_EXACT_ARITHMETIC)
+        r.run()
+        r.synchronize()
+        out.append((r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy(), r.residual_norm()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("case", [
+    dict(fp_samples=2, bp_samples=4, algorithm="sart", lam=0.3, projection_iteration="random"),
+    dict(fp_samples=4, bp_samples=9, algorithm="blockiterative", block_size=4, lam=0.5,
+         projection_iteration="maxangle", use_long_object_compensation=True),
+    dict(fp_samples=3, bp_samples=1, algorithm="sirt", lam=1.0, projection_iteration="identity", mask=True),
+    dict(fp_samples=1, bp_samples=16, algorithm="sart", lam=0.3, projection_iteration="identity",
+         voxel_type="half"),
+])
+def test_oversampling_exact_fused_equals_generic(synthetic, case):
+    """Exact arithmetic with oversampling: the fused kernels trace each sub-ray
+    column once and add its sums in the generic (si, sj) order, and run the
+    generic sample loop of the BP per voxel, so volumes and residual norms are
+    bit-identical to the generic kernels'."""
+    case = dict(case)
+    mask = dev(seeded_mask((64, 64, 64), 9)) if case.pop("mask", False) else None
+    _, tilts, projs = synthetic
+    out = []
+    for flags in (E.ETN_RECON_FLAG_EXACT_ARITHMETIC, E.ETN_RECON_FLAG_GENERIC_KERNELS):
+        r = A.BlockIterativeReconstructionOperator(volume=E.VolumeGeometry.make((64, 64, 64)), base_geometry=TEST_BASE,
+                                                   tilt_angles=tilts[::6], projections=dev(projs[::6]),
+                                                   iterations=2, flags=flags, mask=mask, **case)
         assert r.uses_fused_path() == (flags == E.ETN_RECON_FLAG_EXACT_ARITHMETIC)
         r.run()
         r.synchronize()
