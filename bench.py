@@ -161,8 +161,8 @@ def main():
         % (args.warmup + args.steps, norm, 1e3 * (time.perf_counter() - t_norm)))
 
     block = wl["block_size"] or (1 if wl["algorithm"] == "sart" else N)
-    # the committed PMC traffic summaries cover the unmasked 512^3 metric workload only
-    plain = mask is None and args.fp_samples == 1 and args.bp_samples == 1
+    # the committed PMC traffic summaries cover the 512^3 metric workload (f32 and bf16; f32 masked)
+    plain = args.fp_samples == 1 and args.bp_samples == 1
     roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block, masked=mask is not None,
                                pmc=args.workload == "metric" and plain)
     if args.fp_samples != 1 or args.bp_samples != 1:
@@ -620,7 +620,7 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True)
     achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
     log("dominant kernel %s: %.4f ms average over %d launches -> %.1f GB/s algorithmic"
         % (name, dur_ms, launches, achieved))
-    traffic, source, l2 = measured_traffic(kernel, voxel_type) if pmc else (None, None, None)
+    traffic, source, l2 = measured_traffic(kernel, voxel_type, masked) if pmc else (None, None, None)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
             "l2_hit_rate": l2, "kernel": name,
@@ -628,13 +628,13 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True)
             "kernel_compulsory_bytes_per_launch": int(kernel_bytes), "timed_launches": launches}
 
 
-def measured_traffic(kernel, voxel_type):
+def measured_traffic(kernel, voxel_type, masked=False):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), and its L2 hit rate, when
     the summary exists."""
     name = {0: "fused_plane_kernel", 1: "forward_project_kernel", 2: "back_project_kernel"}[kernel]
     vt = {"float": "f32", "half": "f16", "bfloat16": "bf16"}[voxel_type]
-    path = os.path.join(ROOT, "profiles", "%s_sart_512cubed_121_%s_traffic.md" % (name, vt))
+    path = os.path.join(ROOT, "profiles", "%s_sart_512cubed_121_%s%s_traffic.md" % (name, vt, "_masked" if masked else ""))
     if not os.path.exists(path):
         return None, None, None
     traffic = l2 = None
