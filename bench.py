@@ -133,11 +133,15 @@ def main():
 
     mask = support_mask(nx, ny, nz, dev) if args.mask else None
     flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
+    # the reconstruction runs on a stream of its own, on which the timed
+    # region's HIP events are recorded
+    stream = torch.cuda.Stream()
     r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs,
                                                algorithm=wl["algorithm"], block_size=wl["block_size"],
                                                lam=wl["lam"], projection_iteration=wl["order"],
                                                voxel_type=args.voxel_type, flags=flags, mask=mask,
-                                               fp_samples=args.fp_samples, bp_samples=args.bp_samples)
+                                               fp_samples=args.fp_samples, bp_samples=args.bp_samples,
+                                               stream=stream.cuda_stream)
     fused = r.uses_fused_path()
     for _ in range(args.warmup):
         r.run_single_iteration()
@@ -147,14 +151,19 @@ def main():
 
     # timed region: K iterations, synchronised on both sides
     torch.cuda.synchronize()
+    ev_start, ev_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t_start = time.perf_counter()
+    ev_start.record(stream)
     r.run_iterations(args.steps)
+    ev_end.record(stream)
     r.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    gpu_ms_per_step = ev_start.elapsed_time(ev_end) / args.steps
     ms_per_step = 1e3 * elapsed / args.steps
     gups = V * N * args.steps / elapsed / 1e9
-    log("timed %d iterations: %.2f ms/iteration, %.1f GUPS" % (args.steps, ms_per_step, gups))
+    log("timed %d iterations: %.2f ms/iteration (%.2f ms between the stream's events), %.1f GUPS"
+        % (args.steps, ms_per_step, gpu_ms_per_step, gups))
     t_norm = time.perf_counter()
     norm = r.residual_norm()
     log("residual norm after %d iterations: %.6e (FP64, %.1f ms)"
@@ -164,7 +173,7 @@ def main():
     # the committed PMC traffic summaries cover the 512^3 metric workload (f32 and bf16; f32 masked)
     plain = args.fp_samples == 1 and args.bp_samples == 1
     roofline = kernel_roofline(r, args.voxel_type, fused, V, P, block, masked=mask is not None,
-                               pmc=args.workload == "metric" and plain)
+                               pmc=args.workload == "metric" and plain, gpu_ms_per_step=gpu_ms_per_step, N=N)
     if args.fp_samples != 1 or args.bp_samples != 1:
         roofline["kernel"] += ", FP %dx%d sub-rays, BP %d samples" % (args.fp_samples, args.fp_samples,
                                                                      args.bp_samples)
@@ -174,7 +183,9 @@ def main():
     # HBM traffic rate beside it
     roofline["stream_copy_gbs"] = stream_copy_gbs(dev)
     if roofline.get("traffic"):
+        # the measured HBM bytes per launch at the timed launch rate, against the peak
         roofline["traffic_gbs"] = round(roofline["traffic"] / (roofline["avg_launch_ms"] * 1e-3) / 1e9, 1)
+        roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
 
     if args.calibrate_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -589,21 +600,36 @@ def run_learned(args):
     print(json.dumps(out), flush=True)
 
 
-def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True):
-    """Average duration of the dominant kernel, measured with HIP events around
-    every launch of one extra eager iteration on the reconstruction's stream,
-    and its algorithmic bytes per launch.
+def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True, gpu_ms_per_step=None, N=1):
+    """The dominant kernel's average launch duration and its algorithmic bytes
+    per launch.
 
     Fused plane kernel: one launch back-projects block b-1 and forward projects
     block b (B projections each, B = 1 for SART), i.e. B V voxel updates; its
     algorithmic bytes are the per-block figure of the unfused schedule,
     3 s V + 20 P B (the FP reads the volume, the BP reads and writes it, 20
-    bytes per detector pixel and projection).  Generic FP: s V + 8 P; generic
-    BP: 2 s V + 8 P.  A mask adds one byte per voxel to each volume read
-    (the FP and the BP each read it)."""
+    bytes per detector pixel and projection).  An iteration is N / B such
+    launches' work (its first launch only forward-projects and its last only
+    back-projects), and its launches are all this kernel, so the average launch
+    duration is the timed region's HIP-event time per iteration / (N / B),
+    gaps between launches included.  The fused kernel itself moves
+    2 s V + 28 P B per launch (the compulsory bytes); the rocprofv3 counters
+    give the measured traffic.
+
+    Generic kernels (FP: s V + 8 P; BP: 2 s V + 8 P per launch) share the
+    timed region with the residual kernels, so their duration comes from one
+    extra eager iteration with HIP events around every launch.  A mask adds
+    one byte per voxel to each volume read (the FP and the BP each read it)."""
     s = 4 if voxel_type == "float" else 2
     m = 1 if masked else 0
-    kernel, dur_ms, launches = r.time_kernels()
+    if fused:
+        kernel = 0
+        launches = N / block
+        dur_ms = gpu_ms_per_step / launches
+        source = "timed region: HIP events on the reconstruction stream, per iteration / (N / B) launches"
+    else:
+        kernel, dur_ms, launches = r.time_kernels()
+        source = "one extra eager iteration, HIP events around every launch"
     if kernel == 0:
         name = ("fused_plane_kernel (BP of projection k-1 + FP/residual of projection k)" if block == 1 else
                 "fused_plane_kernel (BP of block b-1 + FP/residual of block b, %d projections each)" % block)
@@ -618,14 +644,17 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True)
     if masked:
         name += ", masked"
     achieved = alg_bytes / (dur_ms * 1e-3) / 1e9
-    log("dominant kernel %s: %.4f ms average over %d launches -> %.1f GB/s algorithmic"
-        % (name, dur_ms, launches, achieved))
-    traffic, source, l2 = measured_traffic(kernel, voxel_type, masked) if pmc else (None, None, None)
+    log("dominant kernel %s: %.4f ms average launch (%s) -> %.1f GB/s algorithmic"
+        % (name, dur_ms, source, achieved))
+    traffic, tsource, l2 = measured_traffic(kernel, voxel_type, masked) if pmc else (None, None, None)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsource,
             "l2_hit_rate": l2, "kernel": name,
-            "avg_launch_ms": round(dur_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes),
-            "kernel_compulsory_bytes_per_launch": int(kernel_bytes), "timed_launches": launches}
+            "avg_launch_ms": round(dur_ms, 5), "avg_launch_source": source,
+            "algorithmic_bytes_per_launch": int(alg_bytes),
+            "kernel_compulsory_bytes_per_launch": int(kernel_bytes),
+            "compulsory_frac": round(kernel_bytes / (dur_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "launches_per_iteration": round(launches, 3)}
 
 
 def measured_traffic(kernel, voxel_type, masked=False):

@@ -46,6 +46,27 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _check(t, name, dtypes, shape):
+    """Rejects a device tensor the C side would read past or misinterpret:
+    the library reads exactly prod(shape) elements of the given type."""
+    torch = _torch()
+    if t is None:
+        return
+    dts = tuple(getattr(torch, d) for d in dtypes)
+    if t.dtype not in dts:
+        raise ValueError("%s must be %s, got %s" % (name, " or ".join(dtypes), t.dtype))
+    if tuple(t.shape) != tuple(int(x) for x in shape):
+        raise ValueError("%s must have shape %s, got %s" % (name, tuple(int(x) for x in shape), tuple(t.shape)))
+
+
+def _volume_shape(vol_geom):
+    return (vol_geom.resolution[2], vol_geom.resolution[1], vol_geom.resolution[0])
+
+
+def _image_shape(g):
+    return (g.resolution[1], g.resolution[0])
+
+
 def _voxel_type_of(t):
     torch = _torch()
     return {torch.float32: ETN_VOXEL_FLOAT32, torch.float16: ETN_VOXEL_FLOAT16,
@@ -114,12 +135,20 @@ def forward_project(vol_geom: VolumeGeometry, volume, g: ScannerGeometry, projec
                     samples=1, subvolume=0, mask=None, stream=None):
     """Adds the subvolume's line integrals into projection / traversal_length
     (float32 device tensors of shape (nv, nu))."""
+    _check(volume, "volume", ("float32", "float16", "bfloat16"), _volume_shape(vol_geom))
+    _check(mask, "mask", ("uint8",), _volume_shape(vol_geom))
+    _check(projection, "projection", ("float32",), _image_shape(g))
+    _check(traversal_length, "traversal_length", ("float32",), _image_shape(g))
     call("etn_forward_project", C.byref(vol_geom), _voxel_type_of(volume), _ptr(volume), _ptr(mask),
          int(subvolume), C.byref(g), int(samples), _ptr(projection), _ptr(traversal_length), _stream(stream))
 
 
 def back_project(vol_geom: VolumeGeometry, volume, g: ScannerGeometry, residual, ray_length, lam=1.0,
                  block_size=1, samples=1, subvolume=ETN_ALL_SUBVOLUMES, mask=None, stream=None):
+    _check(volume, "volume", ("float32", "float16", "bfloat16"), _volume_shape(vol_geom))
+    _check(mask, "mask", ("uint8",), _volume_shape(vol_geom))
+    _check(residual, "residual", ("float32",), _image_shape(g))
+    _check(ray_length, "ray_length", ("float32",), _image_shape(g))
     call("etn_back_project", C.byref(vol_geom), _voxel_type_of(volume), _ptr(volume), _ptr(mask),
          int(subvolume), C.byref(g), int(samples), _ptr(residual), _ptr(ray_length), float(lam),
          int(block_size), _stream(stream))
@@ -127,16 +156,21 @@ def back_project(vol_geom: VolumeGeometry, volume, g: ScannerGeometry, residual,
 
 def compare(real, virtual, residual, stream=None):
     nv, nu = real.shape[-2], real.shape[-1]
+    for name, t in (("real", real), ("virtual", virtual), ("residual", residual)):
+        _check(t, name, ("float32",), (nv, nu))
     call("etn_compare", int(nu), int(nv), _ptr(real), _ptr(virtual), _ptr(residual), _stream(stream))
 
 
 def long_object_compensate(projection, traversal_length, constant_length, stream=None):
     nv, nu = projection.shape[-2], projection.shape[-1]
+    _check(projection, "projection", ("float32",), (nv, nu))
+    _check(traversal_length, "traversal_length", ("float32",), (nv, nu))
     call("etn_long_object_compensate", int(nu), int(nv), _ptr(projection), _ptr(traversal_length),
          float(constant_length), _stream(stream))
 
 
 def compute_ray_length(vol_geom: VolumeGeometry, g: ScannerGeometry, out, stream=None):
+    _check(out, "out", ("float32",), _image_shape(g))
     call("etn_compute_ray_length", C.byref(vol_geom), C.byref(g), _ptr(out), _stream(stream))
 
 
@@ -144,14 +178,23 @@ def ramp_filter(projection, out, stream=None):
     """ConvolutionOperator with WBP's 128-tap RamLak kernel (etn_ramp_filter);
     float32 device tensors of shape (nv, nu)."""
     nv, nu = projection.shape[-2], projection.shape[-1]
+    _check(projection, "projection", ("float32",), (nv, nu))
+    _check(out, "out", ("float32",), (nv, nu))
     call("etn_ramp_filter", int(nu), int(nv), _ptr(projection), _ptr(out), _stream(stream))
 
 
 def sum_squared_difference(a, b, out, stream=None):
+    _check(a, "a", ("float32",), tuple(a.shape))
+    _check(b, "b", ("float32",), tuple(a.shape))
+    _check(out, "out", ("float64",), tuple(out.shape))
+    if out.numel() < 1:
+        raise ValueError("out must hold one double")
     call("etn_sum_squared_difference_f64", _ptr(a), _ptr(b), a.numel(), _ptr(out), _stream(stream))
 
 
 def convert_volume(src, dst, stream=None):
+    if dst.numel() != src.numel():
+        raise ValueError("dst must hold as many voxels as src (%d), got %d" % (src.numel(), dst.numel()))
     call("etn_convert_volume", _ptr(src), _voxel_type_of(src), _ptr(dst), _voxel_type_of(dst), src.numel(),
          _stream(stream))
 
@@ -164,6 +207,10 @@ def _make_desc(*, volume, base_geometry, tilt_angles, projections, algorithm="sa
                x_axis_tilt=0.0, mask=None, flags=0):
     """etn_reconstruction_desc plus the host objects it points into (keep them alive)."""
     tilts = np.ascontiguousarray(tilt_angles, np.float32)
+    # the library reads len(tilts) * nv * nu floats of projections and
+    # nx * ny * nz bytes of mask
+    _check(projections, "projections", ("float32",), (len(tilts),) + _image_shape(base_geometry))
+    _check(mask, "mask", ("uint8",), _volume_shape(volume))
     skip = np.asarray(list(skip_projections) or [0], np.uint32)
     alg = algorithm.encode()
     order = projection_iteration.encode() if projection_iteration else None

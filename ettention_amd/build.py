@@ -23,6 +23,7 @@ OBJ = os.path.join(PKG, "_build")
 SOURCES = [
     "projectors.hip",
     "fused.hip",
+    "fused_inst.hip",
     "unet_kernels.hip",
     "ramp_filter.hip",
     "geometry.cpp",
@@ -39,6 +40,27 @@ SOURCES = [
     "capi.cpp",
     "capi_io.cpp",
 ]
+
+# fused_inst.hip is built once per (arithmetic mode, storage type): each object
+# holds one sixth of the fused plane kernel variants, compiled in parallel.
+FUSED_VARIANTS = [(mode, storage) for mode in ("fast", "exact") for storage in ("F32", "F16", "BF16")]
+
+
+def units():
+    """(source, object stem, extra flags) of every translation unit."""
+    out = []
+    for src in SOURCES:
+        stem = os.path.splitext(src)[0]
+        if src == "fused_inst.hip":
+            for mode, storage in FUSED_VARIANTS:
+                entry = "launch_fused_%s_%s" % (mode, storage)
+                out.append((src, "fused_%s_%s" % (mode, storage.lower()),
+                            ["-DETN_FUSED_FAST=%s" % ("true" if mode == "fast" else "false"),
+                             "-DETN_FUSED_STORAGE=" + storage, "-DETN_FUSED_ENTRY=" + entry]))
+        else:
+            out.append((src, stem, []))
+    return out
+
 
 # -ffp-contract=off: every float expression rounds as written (the projector
 # arithmetic follows the reference's OpenCL C term by term).
@@ -58,9 +80,10 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc()] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(unit):
+    src, stem, extra = unit
+    obj = os.path.join(OBJ, stem + ".o")
+    cmd = [hipcc()] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     noise = "'-packed-fp32-ops' is not a recognized feature"
     err = "\n".join(l for l in p.stderr.splitlines() if noise not in l)
@@ -73,8 +96,11 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    todo = units()
+    # the slowest units first, so that they start before the pool fills up
+    todo.sort(key=lambda u: (u[0] != "fused_inst.hip", u[0] != "unet_kernels.hip"))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(todo))) as ex:
+        objs = list(ex.map(_compile, todo))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     p = subprocess.run(cmd, capture_output=True, text=True)

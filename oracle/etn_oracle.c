@@ -860,6 +860,9 @@ static uint32_t mt_next(mt_state* s)
     y ^= y >> 11; y ^= (y << 7) & 0x9d2c5680u; y ^= (y << 15) & 0xefc60000u; y ^= y >> 18;
     return y;
 }
+uint32_t orc_mt19937_state_words(void) { return (uint32_t)(sizeof(mt_state) / sizeof(uint32_t)); }
+void orc_mt19937_seed(uint32_t* st, uint32_t seed) { mt_seed((mt_state*)st, seed); }
+uint32_t orc_mt19937_next(uint32_t* st) { return mt_next((mt_state*)st); }
 uint32_t orc_mt19937_nth(uint32_t seed, uint32_t n)
 {
     mt_state s; mt_seed(&s, seed);

@@ -84,6 +84,10 @@ float    orc_bf16_to_f32(uint16_t h);
 
 /* ---- projection iteration schemes ---- */
 uint32_t orc_mt19937_nth(uint32_t seed, uint32_t n);
+/* engine state: orc_mt19937_state_words() uint32 words, seeded like mt19937(seed) */
+uint32_t orc_mt19937_state_words(void);
+void     orc_mt19937_seed(uint32_t* state, uint32_t seed);
+uint32_t orc_mt19937_next(uint32_t* state);
 uint32_t orc_boost_uniform_int(uint32_t* mt_state_and_index, uint32_t lo, uint32_t hi);
 int  orc_projection_order(const char* type, uint32_t count, const uint32_t* skip, uint32_t nskip,
                           uint32_t* out, uint32_t* out_count);

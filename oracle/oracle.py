@@ -112,6 +112,12 @@ def lib():
             getattr(L, name).restype = f32
         L.orc_mt19937_nth.argtypes = [u32, u32]
         L.orc_mt19937_nth.restype = u32
+        L.orc_mt19937_state_words.restype = u32
+        L.orc_mt19937_seed.argtypes = [fp, u32]
+        L.orc_mt19937_next.argtypes = [fp]
+        L.orc_mt19937_next.restype = u32
+        L.orc_boost_uniform_int.argtypes = [fp, u32, u32]
+        L.orc_boost_uniform_int.restype = u32
         L.orc_projection_order.argtypes = [C.c_char_p, u32, fp, u32, fp, fp]
         L.orc_projection_order.restype = i32
         L.orc_block_iterative.argtypes = [C.POINTER(Recon), fp, u32]
@@ -265,6 +271,21 @@ def num_threads():
 
 def mt19937_nth(seed, n):
     return int(lib().orc_mt19937_nth(seed, n))
+
+
+class Mt19937:
+    """The oracle's mt19937 engine and its boost::uniform_int<> mapping
+    (the draws of ProjectionSetRandom::reorder)."""
+
+    def __init__(self, seed=5489):
+        self.state = np.zeros(int(lib().orc_mt19937_state_words()), np.uint32)
+        lib().orc_mt19937_seed(_p(self.state), seed)
+
+    def next(self):
+        return int(lib().orc_mt19937_next(_p(self.state)))
+
+    def uniform_int(self, lo, hi):
+        return int(lib().orc_boost_uniform_int(_p(self.state), lo, hi))
 
 
 class ReconSpec:
