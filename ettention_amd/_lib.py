@@ -94,7 +94,7 @@ class ReconstructionDesc(C.Structure):
                 ("init_value", C.c_float), ("base_geometry", ScannerGeometry),
                 ("projection_count", C.c_uint32), ("tilt_angles", C.POINTER(C.c_float)),
                 ("x_axis_tilt", C.c_float), ("projections", C.c_void_p), ("mask", C.c_void_p),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("geometries", C.POINTER(ScannerGeometry))]
 
 
 # (name, restype, argtypes) of every entry point declared in include/etn/etn.h

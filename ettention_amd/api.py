@@ -204,8 +204,10 @@ def convert_volume(src, dst, stream=None):
 def _make_desc(*, volume, base_geometry, tilt_angles, projections, algorithm="sart", block_size=0, iterations=1,
                lam=0.3, use_long_object_compensation=False, fp_samples=1, bp_samples=1,
                projection_iteration="random", skip_projections=(), voxel_type="float", init_value=0.0,
-               x_axis_tilt=0.0, mask=None, flags=0):
-    """etn_reconstruction_desc plus the host objects it points into (keep them alive)."""
+               x_axis_tilt=0.0, mask=None, flags=0, geometries=None):
+    """etn_reconstruction_desc plus the host objects it points into (keep them alive).
+    geometries: optional sequence of ScannerGeometry, one per projection, used
+    instead of rotating base_geometry by the tilt angles."""
     tilts = np.ascontiguousarray(tilt_angles, np.float32)
     # the library reads len(tilts) * nv * nu floats of projections and
     # nx * ny * nz bytes of mask
@@ -235,7 +237,14 @@ def _make_desc(*, volume, base_geometry, tilt_angles, projections, algorithm="sa
     d.projections = _ptr(projections)
     d.mask = _ptr(mask)
     d.flags = int(flags)
-    return d, (tilts, skip, alg, order, projections, mask)
+    geoms = None
+    if geometries is not None:
+        if len(geometries) != len(tilts):
+            raise ValueError("geometries must hold one geometry per projection (%d), got %d"
+                             % (len(tilts), len(geometries)))
+        geoms = (ScannerGeometry * len(geometries))(*geometries)
+        d.geometries = C.cast(geoms, C.POINTER(ScannerGeometry))
+    return d, (tilts, skip, alg, order, projections, mask, geoms)
 
 
 class BlockIterativeReconstructionOperator:
@@ -247,7 +256,8 @@ class BlockIterativeReconstructionOperator:
     def __init__(self, *, volume: VolumeGeometry, base_geometry: ScannerGeometry, tilt_angles, projections,
                  algorithm="sart", block_size=0, iterations=1, lam=0.3, use_long_object_compensation=False,
                  fp_samples=1, bp_samples=1, projection_iteration="random", skip_projections=(),
-                 voxel_type="float", init_value=0.0, x_axis_tilt=0.0, mask=None, flags=0, stream=None):
+                 voxel_type="float", init_value=0.0, x_axis_tilt=0.0, mask=None, flags=0, stream=None,
+                 geometries=None):
         d, self._keep = _make_desc(volume=volume, base_geometry=base_geometry, tilt_angles=tilt_angles,
                                    projections=projections, algorithm=algorithm, block_size=block_size,
                                    iterations=iterations, lam=lam,
@@ -255,7 +265,7 @@ class BlockIterativeReconstructionOperator:
                                    fp_samples=fp_samples, bp_samples=bp_samples,
                                    projection_iteration=projection_iteration, skip_projections=skip_projections,
                                    voxel_type=voxel_type, init_value=init_value, x_axis_tilt=x_axis_tilt,
-                                   mask=mask, flags=flags)
+                                   mask=mask, flags=flags, geometries=geometries)
         self.desc = d
         self.voxel_type = d.voxel_type
         self.shape = (volume.resolution[2], volume.resolution[1], volume.resolution[0])

@@ -30,7 +30,7 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
     else if (algorithm == "blockiterative") general_block_size_ = d.block_size;
     else unsupported("unknown reconstruction algorithm '" + algorithm + "'");
     if (general_block_size_ == 0) fail("Illegal ProjectionBlockSize!");
-    if (d.projection_count == 0 || !d.tilt_angles) fail("No projections given!");
+    if (d.projection_count == 0 || (!d.tilt_angles && !d.geometries)) fail("No projections given!");
     if (!d.projections) fail("projection stack device pointer is null");
     if (d.base_geometry.resolution[0] == 0 || d.base_geometry.resolution[1] == 0) fail("detector resolution is zero");
     pixels_ = (size_t)d.base_geometry.resolution[0] * d.base_geometry.resolution[1];
@@ -38,6 +38,7 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
     desc_.projection_iteration = nullptr;
     desc_.skip_projections = nullptr;
     desc_.tilt_angles = nullptr;
+    desc_.geometries = nullptr;
 
     // ProjectionIteratorFactory::instantiate + setProjectionCount (BlockIterativeProjectionAccess.cpp:27-46)
     projection_set_ = ProjectionIteratorFactory::instantiate(d.projection_iteration ? d.projection_iteration : "random");
@@ -49,11 +50,7 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
         if (i >= d.projection_count) fail("projection index out of range");
     block_count_ = (uint32_t)std::ceil((float)projection_set_->getNumberOfProjections() / (float)general_block_size_);
 
-    // one rotated geometry per physical projection (MRCStack::createScannerGeometry)
-    const SingleAxisTiltRotator rotator{ParallelScannerGeometry(d.base_geometry)};
-    geometries_.reserve(d.projection_count);
-    for (uint32_t i = 0; i < d.projection_count; ++i)
-        geometries_.push_back(rotator.rotated(d.tilt_angles[i], d.x_axis_tilt));
+    geometries_ = projection_geometries(d);
 
     if (stream) {
         stream_ = stream;

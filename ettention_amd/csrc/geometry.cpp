@@ -133,6 +133,24 @@ ParallelScannerGeometry SingleAxisTiltRotator::rotated(float tilt_deg, float x_a
     return g;
 }
 
+std::vector<ParallelScannerGeometry> projection_geometries(const etn_reconstruction_desc& d)
+{
+    std::vector<ParallelScannerGeometry> out;
+    out.reserve(d.projection_count);
+    if (d.geometries) {
+        for (uint32_t i = 0; i < d.projection_count; ++i) {
+            etn_scanner_geometry g = d.geometries[i];
+            g.resolution[0] = d.base_geometry.resolution[0];
+            g.resolution[1] = d.base_geometry.resolution[1];
+            out.emplace_back(g);
+        }
+        return out;
+    }
+    const SingleAxisTiltRotator rotator{ParallelScannerGeometry(d.base_geometry)};
+    for (uint32_t i = 0; i < d.projection_count; ++i) out.push_back(rotator.rotated(d.tilt_angles[i], d.x_axis_tilt));
+    return out;
+}
+
 std::vector<VoxelSample> bp_sample_pattern(const ParallelScannerGeometry& g, const Vec3f& vs, uint32_t samples)
 {
     if (samples == 0) fail("zero samples not supported for BackProjection");

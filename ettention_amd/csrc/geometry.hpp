@@ -93,6 +93,13 @@ private:
     ParallelScannerGeometry base_;
 };
 
+// The geometry of every projection of a reconstruction: desc.geometries when
+// given (ImageStackDatasource::createScannerGeometry, ImageStackDatasource.h:26),
+// otherwise the base geometry rotated by each tilt angle and the x-axis tilt
+// (MRCStack::createScannerGeometry, MRCStack.cpp:281-293).  Every geometry
+// takes the detector resolution of desc.base_geometry.
+std::vector<ParallelScannerGeometry> projection_geometries(const etn_reconstruction_desc& d);
+
 // One BP sample: offset from the voxel centre and the in-voxel chord length.
 struct VoxelSample {
     float dx, dy, dz, length;

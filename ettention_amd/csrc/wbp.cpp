@@ -62,16 +62,14 @@ dev::RampFilterParams WeightedBackProjection::filterParams(uint32_t nu, uint32_t
 WeightedBackProjection::WeightedBackProjection(const etn_reconstruction_desc& d, hipStream_t stream)
     : count_(d.projection_count), lambda_(d.lambda), projections_(d.projections)
 {
-    if (d.projection_count == 0 || !d.tilt_angles) fail("No projections given!");
+    if (d.projection_count == 0 || (!d.tilt_angles && !d.geometries)) fail("No projections given!");
     if (!d.projections) fail("projection stack device pointer is null");
     const uint32_t nu = d.base_geometry.resolution[0], nv = d.base_geometry.resolution[1];
     if (nu == 0 || nv == 0) fail("detector resolution is zero");
     pixels_ = (size_t)nu * nv;
     params_ = filterParams(nu, nv, kFilterSize);
 
-    const SingleAxisTiltRotator rotator{ParallelScannerGeometry(d.base_geometry)};
-    geometries_.reserve(count_);
-    for (uint32_t i = 0; i < count_; ++i) geometries_.push_back(rotator.rotated(d.tilt_angles[i], d.x_axis_tilt));
+    geometries_ = projection_geometries(d);
 
     if (stream) {
         stream_ = stream;

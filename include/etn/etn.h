@@ -197,6 +197,13 @@ typedef struct etn_reconstruction_desc {
     const float* projections;           /* device, projection_count * nu * nv, stack order */
     const uint8_t* mask;                /* device prior-knowledge mask or NULL */
     uint32_t flags;                     /* ETN_RECON_FLAG_* */
+    /* host, projection_count entries, or NULL: the geometry of each projection
+     * as ImageStackDatasource::createScannerGeometry returns it
+     * (ImageStackDatasource.h:26; MRCStack.cpp:281-293,
+     * ImageStackDirectoryDatasource.cpp:268-276).  When given, it replaces
+     * rotating base_geometry by tilt_angles and x_axis_tilt, and tilt_angles may
+     * be NULL; base_geometry.resolution still gives the detector size. */
+    const etn_scanner_geometry* geometries;
 } etn_reconstruction_desc;
 
 /* The block-iterative ids select BlockIterativeReconstructionOperator

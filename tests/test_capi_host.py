@@ -32,6 +32,31 @@ def test_library_exports_every_declared_symbol():
     assert sorted(n for n, _, _ in _lib.SIGNATURES) == names
 
 
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """The ctypes mirrors in _lib.py have the C compiler's field offsets and sizes."""
+    import subprocess
+    exe = str(tmp_path / "desc_layout")
+    here = os.path.dirname(os.path.abspath(__file__))
+    subprocess.run(["gcc", "-std=c99", "-I" + os.path.dirname(os.path.dirname(HEADER)),
+                    os.path.join(here, "native", "desc_layout.c"), "-o", exe], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    mirrors = {"etn_reconstruction_desc": _lib.ReconstructionDesc, "etn_scanner_geometry": _lib.ScannerGeometry,
+               "etn_volume_geometry": _lib.VolumeGeometry}
+    checked = 0
+    for line in out:
+        if not line:
+            continue
+        struct, field, value = line.split()
+        cls = mirrors[struct]
+        if field == "sizeof":
+            assert C.sizeof(cls) == int(value), struct
+        else:
+            name = "lambda_" if field == "lambda" else field
+            assert getattr(cls, name).offset == int(value), (struct, field)
+        checked += 1
+    assert checked == 24
+
+
 def _to_oracle(g):
     return O.Scanner.make(g.source, g.detector, g.horizontal_pitch, g.vertical_pitch, g.resolution)
 
