@@ -117,6 +117,7 @@ SIGNATURES = [
     ("etn_compute_ray_length", _i32, [_VG, _SG, _vp, _vp]),
     ("etn_ramp_filter", _i32, [_u32, _u32, _vp, _vp, _vp]),
     ("etn_sum_squared_difference_f64", _i32, [_vp, _vp, C.c_size_t, _vp, _vp]),
+    ("etn_sum_squared_difference_f64_noalloc", _i32, [_vp, _vp, C.c_size_t, _vp, _vp]),
     ("etn_convert_volume", _i32, [_vp, _i32, _vp, _i32, C.c_size_t, _vp]),
     ("etn_reconstruction_create", _i32, [C.POINTER(ReconstructionDesc), _vp, C.POINTER(_vp)]),
     ("etn_reconstruction_destroy", _i32, [_vp]),

@@ -192,6 +192,18 @@ def sum_squared_difference(a, b, out, stream=None):
     call("etn_sum_squared_difference_f64", _ptr(a), _ptr(b), a.numel(), _ptr(out), _stream(stream))
 
 
+def sum_squared_difference_noalloc(a, b, out, stream=None):
+    """sum_squared_difference without device allocation: the partial sums use
+    the library's static per-GPU scratch, so overlapping calls on one GPU race."""
+    _check(a, "a", ("float32",), tuple(a.shape))
+    if b is not None:
+        _check(b, "b", ("float32",), tuple(a.shape))
+    _check(out, "out", ("float64",), tuple(out.shape))
+    if out.numel() < 1:
+        raise ValueError("out must hold one double")
+    call("etn_sum_squared_difference_f64_noalloc", _ptr(a), _ptr(b), a.numel(), _ptr(out), _stream(stream))
+
+
 def convert_volume(src, dst, stream=None):
     if dst.numel() != src.numel():
         raise ValueError("dst must hold as many voxels as src (%d), got %d" % (src.numel(), dst.numel()))

@@ -267,6 +267,24 @@ etn_status etn_sum_squared_difference_f64(const float* a, const float* b, size_t
     });
 }
 
+etn_status etn_sum_squared_difference_f64_noalloc(const float* a, const float* b, size_t count, double* out_device,
+                                                  void* stream)
+{
+    return guarded([&] {
+        need(a, "input");
+        need(out_device, "output");
+        if (count > ETN_SSD_NOALLOC_MAX_COUNT) etn::fail("allocation-free residual norm: count above the maximum");
+        hipStream_t s = as_stream(stream);
+        // the scratch of the GPU the stream runs on (the current one for the null stream)
+        int device = 0;
+        if (s) ETN_HIP(hipStreamGetDevice(s, &device));
+        else ETN_HIP(hipGetDevice(&device));
+        static_assert(etn::dev::kSsdBlocks * sizeof(double) <= ETN_SSD_NOALLOC_SCRATCH_BYTES,
+                      "the partial sums must fit the static scratch");
+        etn::dev::launch_sum_squared_difference(a, b, count, etn::dev::ssd_static_scratch(device), out_device, s);
+    });
+}
+
 etn_status etn_convert_volume(const void* src, etn_voxel_type src_type, void* dst, etn_voxel_type dst_type,
                               size_t count, void* stream)
 {

@@ -48,6 +48,9 @@ void launch_ray_length(const float bmin[3], const float bmax[3], const RayParams
 constexpr uint32_t kSsdBlocks = 512;
 void launch_sum_squared_difference(const float* a, const float* b, size_t count, double* partials, double* out,
                                    hipStream_t stream);
+// Device address of `device`'s static partial-sum array (kSsdBlocks doubles),
+// looked up once per device with that device current.
+double* ssd_static_scratch(int device);
 void launch_convert(const void* src, etn_voxel_type st, void* dst, etn_voxel_type dt, size_t count, hipStream_t stream);
 void launch_fill(void* dst, etn_voxel_type t, float value, size_t count, hipStream_t stream);
 

@@ -3,6 +3,9 @@
 // voxel (back projection), for any parallel geometry, any subvolume split and
 // any sample count.  The fused plane kernels (fused.hip) take over the common
 // single-axis case; these serve everything else and the per-kernel C ABI.
+#include <mutex>
+#include <vector>
+
 #include "device_math.hpp"
 #include "common.hpp"
 
@@ -260,6 +263,34 @@ void launch_sum_squared_difference(const float* a, const float* b, size_t count,
     ssd_partial_kernel<<<kSsdBlocks, kSsdThreads, 0, stream>>>(a, b, count, partials);
     ssd_final_kernel<<<1, kSsdThreads, 0, stream>>>(partials, kSsdBlocks, out);
     check_launch("residual norm launch");
+}
+
+// Scratch of the allocation-free residual norm: the kSsdBlocks FP64 partial
+// sums of ssd_partial_kernel (4 KiB), one copy per GPU, held by each GPU the
+// library runs on until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host it names a stand-in, not device memory): it
+// looks up each device's address once, with that device current, and uses it.
+static __device__ __align__(256) double scratch[kSsdBlocks];
+
+double* ssd_static_scratch(int device)
+{
+    static std::mutex lock;
+    static std::vector<double*> address;   // per device, filled on first use
+    std::lock_guard<std::mutex> guard(lock);
+    if (device < 0) fail("residual norm scratch: invalid device");
+    if ((size_t)device >= address.size()) address.resize((size_t)device + 1, nullptr);
+    if (!address[device]) {
+        int current = 0;
+        ETN_HIP(hipGetDevice(&current));
+        if (current != device) ETN_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (current != device) ETN_HIP(hipSetDevice(current));
+        ETN_HIP(e);
+        if (!p) fail("residual norm scratch: no device address");
+        address[device] = static_cast<double*>(p);
+    }
+    return address[device];
 }
 
 void launch_convert(const void* src, etn_voxel_type st, void* dst, etn_voxel_type dt, size_t count, hipStream_t stream)

@@ -152,6 +152,22 @@ etn_status etn_ramp_filter(uint32_t nu, uint32_t nv, const float* projection, fl
 etn_status etn_sum_squared_difference_f64(const float* a, const float* b, size_t count,
                                           double* out_device, void* stream);
 
+/* Allocation-free variant of etn_sum_squared_difference_f64, with the same
+ * kernels and the same result bit for bit.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its FP64
+ * partial sums live in a static device array of ETN_SSD_NOALLOC_SCRATCH_BYTES
+ * per GPU (its address is looked up on the first call on each GPU), so it
+ * calls no hipMalloc/hipFree.  The partial count is fixed, so the scratch
+ * bounds nothing but count <= ETN_SSD_NOALLOC_MAX_COUNT (2^40 elements, 4 TiB
+ * of floats; larger counts are refused).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (different streams, host
+ * threads) race, so serialise them. */
+#define ETN_SSD_NOALLOC_SCRATCH_BYTES 4096u
+#define ETN_SSD_NOALLOC_MAX_COUNT ((size_t)1 << 40)
+etn_status etn_sum_squared_difference_f64_noalloc(const float* a, const float* b, size_t count,
+                                                  double* out_device, void* stream);
+
 /* Storage conversion helpers for reduced-precision volumes (device to device). */
 etn_status etn_convert_volume(const void* src, etn_voxel_type src_type, void* dst,
                               etn_voxel_type dst_type, size_t count, void* stream);
