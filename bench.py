@@ -601,20 +601,20 @@ def run_learned(args):
 
 
 def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True, gpu_ms_per_step=None, N=1):
-    """The dominant kernel's average launch duration and its algorithmic bytes
-    per launch.
+    """The dominant kernel's algorithmic bytes against its measured time.
 
-    Fused plane kernel: one launch back-projects block b-1 and forward projects
-    block b (B projections each, B = 1 for SART), i.e. B V voxel updates; its
-    algorithmic bytes are the per-block figure of the unfused schedule,
-    3 s V + 20 P B (the FP reads the volume, the BP reads and writes it, 20
-    bytes per detector pixel and projection).  An iteration is N / B such
-    launches' work (its first launch only forward-projects and its last only
-    back-projects), and its launches are all this kernel, so the average launch
-    duration is the timed region's HIP-event time per iteration / (N / B),
-    gaps between launches included.  The fused kernel itself moves
-    2 s V + 28 P B per launch (the compulsory bytes); the rocprofv3 counters
-    give the measured traffic.
+    Fused plane kernel: an iteration's launches are all this kernel.  Its
+    algorithmic bytes per iteration are the unfused schedule's (the figure the
+    roofline is defined by): per block of B projections 3 s V + 20 P B (the FP
+    reads the volume, the BP reads and writes it, 20 bytes per detector pixel
+    and projection), summed over the ceil(N / B) blocks.  The time is the
+    timed region's HIP-event time per iteration, so achieved = bytes per
+    iteration / time per iteration; the average launch is that time over the
+    launches one iteration makes (counted by the library; for SART N + 1: the
+    first launch only forward-projects, the last only back-projects).  A
+    launch that back-projects and forward-projects moves 2 s V + 28 P B' bytes
+    itself (B' projections each way: the compulsory bytes); the rocprofv3
+    counters give the measured traffic.
 
     Generic kernels (FP: s V + 8 P; BP: 2 s V + 8 P per launch) share the
     timed region with the residual kernels, so their duration comes from one
@@ -622,24 +622,25 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True,
     one byte per voxel to each volume read (the FP and the BP each read it)."""
     s = 4 if voxel_type == "float" else 2
     m = 1 if masked else 0
+    kernel, eager_ms, launches = r.time_kernels()
     if fused:
-        kernel = 0
-        launches = N / block
+        assert kernel == 0
+        blocks = -(-N // block)
+        alg_iter = blocks * (3 * s + 2 * m) * V + 20 * P * N
         dur_ms = gpu_ms_per_step / launches
-        source = "timed region: HIP events on the reconstruction stream, per iteration / (N / B) launches"
-    else:
-        kernel, dur_ms, launches = r.time_kernels()
-        source = "one extra eager iteration, HIP events around every launch"
-    if kernel == 0:
+        alg_bytes = alg_iter / launches
+        kernel_bytes = (2 * s + m) * V + 28 * P * N / launches   # projections per launch, on average
+        source = ("timed region: HIP events on the reconstruction stream, time per iteration / %d launches"
+                  % launches)
         name = ("fused_plane_kernel (BP of projection k-1 + FP/residual of projection k)" if block == 1 else
-                "fused_plane_kernel (BP of block b-1 + FP/residual of block b, %d projections each)" % block)
-        alg_bytes = (3 * s + 2 * m) * V + 20 * P * block
-        kernel_bytes = (2 * s + m) * V + 28 * P * block
-    elif kernel == 1:
-        name, alg_bytes = "forward_project_kernel (generic DDA)", (s + m) * V + 8 * P
-        kernel_bytes = alg_bytes
+                "fused_plane_kernel (BPs of block b-1 + FPs/residuals of block b, %d projections per block)" % block)
     else:
-        name, alg_bytes = "back_project_kernel (generic)", (2 * s + m) * V + 8 * P
+        dur_ms = eager_ms
+        source = "one extra eager iteration, HIP events around every launch"
+        if kernel == 1:
+            name, alg_bytes = "forward_project_kernel (generic DDA)", (s + m) * V + 8 * P
+        else:
+            name, alg_bytes = "back_project_kernel (generic)", (2 * s + m) * V + 8 * P
         kernel_bytes = alg_bytes
     if masked:
         name += ", masked"
@@ -654,7 +655,7 @@ def kernel_roofline(r, voxel_type, fused, V, P, block=1, masked=False, pmc=True,
             "algorithmic_bytes_per_launch": int(alg_bytes),
             "kernel_compulsory_bytes_per_launch": int(kernel_bytes),
             "compulsory_frac": round(kernel_bytes / (dur_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "launches_per_iteration": round(launches, 3)}
+            "launches_per_iteration": launches}
 
 
 def measured_traffic(kernel, voxel_type, masked=False):

@@ -307,14 +307,16 @@ double BlockIterativeReconstructionOperator::timeKernels(int* kernel, uint32_t* 
         ensureTuned();
         *kernel = 0;
         double total = 0.0;
-        uint32_t n = 0;
-        for (const auto& t : fused_->timeIteration(stream_))
+        uint32_t n = 0, all = 0;
+        for (const auto& t : fused_->timeIteration(stream_)) {
+            ++all;
             if (t.steady) {
                 total += t.ms;
                 ++n;
             }
+        }
         fused_->check(stream_);
-        *launches = n;
+        *launches = all;
         return n ? total / n : 0.0;
     }
     // one eager iteration with an event pair around every FP and BP launch

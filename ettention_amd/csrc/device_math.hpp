@@ -59,13 +59,16 @@ struct F16 {   // vload_half / vstore_half (round to nearest even), HalfFloatSup
 };
 struct BF16 {  // bfloat16, round to nearest even
     using T = uint16_t;
-    static __device__ __forceinline__ uint16_t pack(float f)
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    // two values in one v_cvt_pk_bf16_f32 (gfx950's conversion, round to
+    // nearest even; a NaN stays a NaN): a in the low half, b in the high half
+    static __device__ __forceinline__ uint32_t pack2(float a, float b)
     {
-        uint32_t u = __float_as_uint(f);
-        if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (uint16_t)(u >> 16);
+        const f32x2 v = {a, b};
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
     }
+    static __device__ __forceinline__ uint16_t pack(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
     static __device__ __forceinline__ float unpack(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
     template <typename P>
     static __device__ __forceinline__ float load(P p, size_t i) { return unpack(p[i]); }

@@ -58,6 +58,7 @@ struct MemberPlan {
     int row_floats = 0;                // LDS floats of one plane's staged rows
     const uint8_t* mask = nullptr;     // prior-knowledge mask (device), or null
     int threads = dev::kMaxThreads;    // workgroup shape of the engine's launches
+    int vbytes = 4;                    // bytes of a stored voxel (the 512-thread shape's LDS array depends on it)
     int fp_n = 1, fp_shift = 0;        // forward sub-rays per pixel axis; log2 of ray slots (columns si) per pixel
     int bp_samples = 1, rowsets = 1;   // back projection samples per voxel; pattern rows
     bool oversampled() const { return fp_n != 1 || bp_samples != 1; }
@@ -84,10 +85,10 @@ int max_band(const MemberPlan& m, int planes, int threads)
     if ((planes * m.rows_per_plane * m.nu << m.fp_shift) > threads * dev::kMaxRaysPerThread) return 0;
     if (m.oversampled() && threads != dev::kMaxThreads) return 0;   // oversampled kernels: 512 threads only
     const bool mask = m.mask != nullptr;
-    const int masked = mask ? dev::shape_band_floats(threads, true) : 0;
-    int band = std::min(m.nz, (dev::shape_lds_floats(threads) - masked - 3 -
+    const int masked = mask ? dev::shape_band_floats(threads, true, m.vbytes) : 0;
+    int band = std::min(m.nz, (dev::shape_lds_floats(threads, m.vbytes) - masked - 3 -
                                planes * m.rowsets * (m.row_floats + 2)) / (planes * m.nx));
-    band = std::min(band, dev::shape_band_floats(threads, mask) / (planes * m.nx));
+    band = std::min(band, dev::shape_band_floats(threads, mask, m.vbytes) / (planes * m.nx));
     return std::max(band, 0);
 }
 
@@ -96,8 +97,8 @@ int max_band(const MemberPlan& m, int planes) { return max_band(m, planes, m.thr
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
-           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr, m.threads, m.rowsets) <=
-               dev::shape_lds_floats(m.threads);
+           dev::fused_lds_floats(t.planes, t.band, m.nx, m.row_floats, m.mask != nullptr, m.threads, m.rowsets,
+                                 m.vbytes) <= dev::shape_lds_floats(m.threads, m.vbytes);
 }
 
 // Without timing: the fewest planes whose rays fill a workgroup, the tallest band.
@@ -285,12 +286,13 @@ bool plan_member(BlockIterativeReconstructionOperator& op, const etn_reconstruct
     out.row_floats = out.row_len * (exact ? 8 : 4);
     out.mask = d.mask;
     out.threads = dev::kMaxThreads;
+    out.vbytes = (int)voxel_bytes(d.voxel_type);
     out.fp_n = fp_n;
     out.fp_shift = fp_shift;
     out.bp_samples = bp_s;
     out.rowsets = bp_pd;
-    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr, out.threads, out.rowsets) >
-        dev::shape_lds_floats(out.threads))
+    if (dev::fused_lds_floats(1, 1, nx, out.row_floats, out.mask != nullptr, out.threads, out.rowsets,
+                              out.vbytes) > dev::shape_lds_floats(out.threads, out.vbytes))
         return false;   // a z row and the rows fit
     if (max_band(out, 1) < 1) return false;   // not even one z row of one plane fits
     out.ndead = (int)dead.size();
@@ -561,7 +563,7 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
                                                threads_));
             l.lds = std::max(l.lds, sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx,
                                                                                    m.row_floats, mask_, threads_,
-                                                                                   m.rowsets));
+                                                                                   m.rowsets, m.vbytes));
             l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
         }
         l.groups = (int)(work.size() - l.work);
@@ -649,7 +651,7 @@ double FusedEngineImpl::residualNorm(size_t member, double* scratch, hipStream_t
     double* total = scratch + dev::kSsdBlocks;
     ETN_HIP(hipMemsetAsync(total, 0, sizeof(double), stream));
     const size_t lds = sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx, m.row_floats,
-                                                                      mask_, threads_, m.rowsets);
+                                                                      mask_, threads_, m.rowsets, m.vbytes);
     size_t pass = 0;
     for (size_t c = 0; c < chunk_end.size(); ++c) {
         for (; pass < chunk_end[c]; ++pass) {
@@ -929,7 +931,8 @@ void FusedBackProjectorImpl::backProject(const std::vector<uint32_t>& indices,
     ETN_HIP(hipMemcpyAsync(dwork.get(), work.data(), dwork.bytes(), hipMemcpyHostToDevice, stream));
     ETN_HIP(hipMemcpyAsync(dtables.get(), tables.data(), dtables.bytes(), hipMemcpyHostToDevice, stream));
     const size_t lds = sizeof(float) * (size_t)dev::fused_lds_floats(m_.tile.planes, m_.tile.band, m_.nx,
-                                                                      m_.row_floats, false, m_.threads, m_.rowsets);
+                                                                      m_.row_floats, false, m_.threads, m_.rowsets,
+                                                                      m_.vbytes);
     std::vector<hipEvent_t> ev(launch_ms ? 2 * passes.size() : 0);
     for (auto& e : ev) ETN_HIP(hipEventCreate(&e));
     for (size_t k = 0; k < passes.size(); ++k) {
@@ -981,10 +984,11 @@ std::unique_ptr<FusedBackProjector> FusedBackProjector::create(GpuVolume* volume
     m.rows_per_plane = 1;
     m.row_floats = m.row_len * 4;
     m.threads = dev::kMaxThreads;
+    m.vbytes = (int)voxel_bytes(volume->voxel_type());
     m.bp_samples = bp_s;
     m.rowsets = pd;
-    if (dev::fused_lds_floats(1, 1, m.nx, m.row_floats, false, m.threads, m.rowsets) >
-        dev::shape_lds_floats(m.threads) || max_band(m, 1) < 1)
+    if (dev::fused_lds_floats(1, 1, m.nx, m.row_floats, false, m.threads, m.rowsets, m.vbytes) >
+        dev::shape_lds_floats(m.threads, m.vbytes) || max_band(m, 1) < 1)
         return nullptr;
     m.tile = default_tile(m);
     return std::make_unique<FusedBackProjectorImpl>(volume, geometries, std::move(m));

@@ -22,6 +22,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 #include "device_math.hpp"
 #include "fused.hpp"
@@ -37,29 +38,42 @@ constexpr int kMaxFusedFpPerAxis = 4;   // forward sub-rays per pixel axis (over
 constexpr int kMaxFusedBpSamples = 16;  // back projection samples per voxel (oversampled kernels)
 constexpr int kRowPad = 2;           // zero taps on each side of a staged detector row
 
-// Workgroup shapes.  512 threads with a 64 KiB LDS array run two workgroups
-// per CU; 256 threads with 40 KiB run four, whose independent barrier phases
-// overlap better when a pass carries many projections, at half the band.
-// kBand: band voxels (the vector path prefetches 6 groups of 4 per thread).
-// With a prior-knowledge mask the band holds kBand / 2 voxels and the voxels'
-// mask weights follow at LDS offset kBand / 2 (floats).
-template <int kThreads>
+// Workgroup shapes.  512 threads run two workgroups per CU; 256 threads with
+// 40 KiB run four, whose independent barrier phases overlap better when a pass
+// carries many projections, at half the band.
+// kBand: band voxels; the vector path prefetches kBand / (4 kThreads) groups
+// of 4 per thread in registers: 6 for f32 (24 VGPRs); 16-bit storage holds a
+// group in 2 VGPRs, so its 512-thread shape takes 8 groups (taller bands,
+// fewer barriers per sweep) in an 80 KiB array (gfx950 gives a workgroup up to
+// 160 KiB; two of them fill a CU).  With a prior-knowledge mask the band holds
+// kBand / 2 voxels and the voxels' mask weights follow at LDS offset kBand / 2
+// (floats).  kBytes: bytes of a stored voxel.
+template <int kThreads, int kBytes>
 struct FusedShape;
 template <>
-struct FusedShape<512> {
+struct FusedShape<512, 4> {
     static constexpr int kLds = 16384;
     static constexpr int kBand = 12288;
 };
 template <>
-struct FusedShape<256> {
+struct FusedShape<512, 2> {
+    static constexpr int kLds = 20480;
+    static constexpr int kBand = 16384;
+};
+template <int kBytes>
+struct FusedShape<256, kBytes> {
     static constexpr int kLds = 10240;
     static constexpr int kBand = 6144;
 };
 constexpr int kMaxThreads = 512;
-__host__ __device__ inline int shape_lds_floats(int threads) { return threads == 256 ? FusedShape<256>::kLds : FusedShape<512>::kLds; }
-__host__ __device__ inline int shape_band_floats(int threads, bool mask)
+__host__ __device__ inline int shape_lds_floats(int threads, int vbytes)
 {
-    const int b = threads == 256 ? FusedShape<256>::kBand : FusedShape<512>::kBand;
+    return threads == 256 ? FusedShape<256, 4>::kLds : vbytes == 2 ? FusedShape<512, 2>::kLds : FusedShape<512, 4>::kLds;
+}
+__host__ __device__ inline int shape_band_floats(int threads, bool mask, int vbytes)
+{
+    const int b = threads == 256 ? FusedShape<256, 4>::kBand
+                                 : vbytes == 2 ? FusedShape<512, 2>::kBand : FusedShape<512, 4>::kBand;
     return mask ? b / 2 : b;
 }
 
@@ -120,15 +134,15 @@ struct FusedWork {
 };
 
 // LDS floats a workgroup of `threads` uses: the band (with a mask: the band,
-// at offset shape_band_floats(threads, true) the band's mask weights), then
+// at offset shape_band_floats(threads, true, vbytes) the band's mask weights), then
 // (16-byte aligned) the staged detector records of each plane and row set
 // (row_floats per row set: 8 floats per padded column with exact arithmetic,
 // 4 with fast; one row set per BP pattern row) and a (m4 py, ty) pair per
 // plane and row set.
 __host__ __device__ inline int fused_lds_floats(int planes, int band, int nx, int row_floats, bool mask, int threads,
-                                                int rowsets)
+                                                int rowsets, int vbytes)
 {
-    return (mask ? shape_band_floats(threads, true) : 0) + ((band * planes * nx + 3) & ~3) +
+    return (mask ? shape_band_floats(threads, true, vbytes) : 0) + ((band * planes * nx + 3) & ~3) +
            planes * rowsets * (row_floats + 2);
 }
 
@@ -289,8 +303,13 @@ struct GroupIo16 {   // 4 x 16-bit voxels as one 8-byte access
     static __device__ __forceinline__ void store(ETN_GLOBAL typename S16::T* p, size_t i, const float* f)
     {
         V v;
-        typename S16::T* t = reinterpret_cast<typename S16::T*>(&v);
-        for (int k = 0; k < 4; ++k) S16::store(t, k, f[k]);
+        if constexpr (std::is_same<S16, BF16>::value) {
+            v.x = BF16::pack2(f[0], f[1]);
+            v.y = BF16::pack2(f[2], f[3]);
+        } else {
+            typename S16::T* t = reinterpret_cast<typename S16::T*>(&v);
+            for (int k = 0; k < 4; ++k) S16::store(t, k, f[k]);
+        }
         *reinterpret_cast<ETN_GLOBAL uint2*>(p + i) = v;
     }
 };
@@ -348,7 +367,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                        const FusedPass* const* __restrict__ passes)
 {
     using IO = GroupIo<S, kVec>;
-    using Shape = FusedShape<kThreads>;
+    using Shape = FusedShape<kThreads, (int)sizeof(typename S::T)>;
     constexpr int G = IO::G;
     constexpr int kMaskOff = Shape::kBand / 2;          // LDS floats from a voxel to its mask weight
     constexpr int kBandCap = kMask ? kMaskOff : Shape::kBand;
@@ -874,7 +893,8 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
 template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads>
 void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
 {
-    if (l.lds > sizeof(float) * (size_t)FusedShape<kThreads>::kLds) fail("fused plane kernel: tile exceeds the LDS array");
+    if (l.lds > sizeof(float) * (size_t)FusedShape<kThreads, (int)sizeof(typename S::T)>::kLds)
+        fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kThreads);
     switch (l.rpt) {
         case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;

@@ -253,8 +253,10 @@ etn_status etn_reconstruction_uses_fused_path(etn_reconstruction* r, int* out);
  * CLAbstractionLayer.cpp:295-329): runs ONE more iteration eagerly with HIP
  * events around every launch on the reconstruction's stream and reports the
  * average duration of the kernel that dominates it: *kernel = 0 fused plane
- * kernel (launches carrying a back and a forward projection), 1 generic forward
- * projection, 2 generic back projection.  Advances the reconstruction. */
+ * kernel (average over the launches carrying a back and a forward projection;
+ * *launches = every launch of the iteration), 1 generic forward projection,
+ * 2 generic back projection (*launches = that kernel's launches).  Advances
+ * the reconstruction. */
 etn_status etn_reconstruction_time_kernels(etn_reconstruction* r, int* kernel, double* average_ms,
                                            uint32_t* launches);
 /* Extension (no reference counterpart): the fused kernels' tile, i.e. volume

@@ -109,6 +109,24 @@ def test_fp_reduced_precision_volume(phantom, dtype, vt):
     assert rel_l2(host(p), op) <= TOL
 
 
+@pytest.mark.parametrize("dtype,fn", [("bfloat16", "orc_f32_to_bf16_rne"), ("float16", "orc_f32_to_f16_rte")])
+def test_storage_rounding_bit_exact(dtype, fn):
+    """The kernels' rounding to 16-bit storage (etn_convert_volume uses the
+    storage types' store) against the oracle's, bit for bit, on ties,
+    denormals, overflow and random bit patterns; NaN stays NaN."""
+    from tests.test_oracle_rounding import rounding_cases
+    f = rounding_cases(n_random=50000, seed=11)
+    out = torch.empty(f.size, dtype=getattr(torch, dtype), device="cuda")
+    A.convert_volume(dev(f), out)
+    got = host(out.view(torch.int16)).view(np.uint16)
+    want = np.array([getattr(O.lib(), fn)(float(x)) for x in f], dtype=np.uint16)
+    np.testing.assert_array_equal(got, want)
+    nan = torch.full((4,), float("nan"), dtype=torch.float32, device="cuda")
+    out = torch.empty(4, dtype=getattr(torch, dtype), device="cuda")
+    A.convert_volume(nan, out)
+    assert bool(torch.isnan(out.float()).all())
+
+
 @pytest.mark.parametrize("tag,angle", [("z", 0.0), ("y", 90.0), ("45", 45.0)])
 def test_bp_matches_oracle_and_fixture(tag, angle):
     proj, ref = bp_fixture(tag)
