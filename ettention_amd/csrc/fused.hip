@@ -59,6 +59,7 @@ struct MemberPlan {
     const uint8_t* mask = nullptr;     // prior-knowledge mask (device), or null
     int threads = dev::kMaxThreads;    // workgroup shape of the engine's launches
     int vbytes = 4;                    // bytes of a stored voxel (the 512-thread shape's LDS array depends on it)
+    int rpt_cap = dev::kMaxRaysPerThread;  // rays per thread a pass may give its forward projections
     int fp_n = 1, fp_shift = 0;        // forward sub-rays per pixel axis; log2 of ray slots (columns si) per pixel
     int bp_samples = 1, rowsets = 1;   // back projection samples per voxel; pattern rows
     bool oversampled() const { return fp_n != 1 || bp_samples != 1; }
@@ -314,7 +315,7 @@ void build_passes(MemberPlan& mb)
 {
     BlockIterativeReconstructionOperator& op = *mb.op;
     mb.passes.clear();
-    const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, mb.threads * dev::kMaxRaysPerThread /
+    const int max_fp = std::max(1, std::min(dev::kMaxFusedFp, mb.threads * mb.rpt_cap /
                                                                   ((mb.tile.planes * mb.rows_per_plane * mb.nu)
                                                                    << mb.fp_shift)));
     auto image = [&](uint32_t parity, uint32_t slot, int which) {
@@ -426,8 +427,11 @@ private:
     void enqueue(const Schedule& s, size_t first, size_t last, hipStream_t stream, std::vector<hipEvent_t>* ev) const;
     bool threadsFit(int threads) const;
     void useThreads(int threads);
+    void useRptCap(int cap);
+    size_t passCount() const;
     void tuneTiles(hipStream_t stream);
     float timeLaunches(const Schedule& s, hipStream_t stream) const;
+    float timeSchedule(const Schedule& s, hipStream_t stream) const;
 
     std::vector<MemberPlan> members_;
     etn_voxel_type vt_;
@@ -735,6 +739,18 @@ void FusedEngineImpl::useThreads(int threads)
     }
 }
 
+void FusedEngineImpl::useRptCap(int cap)
+{
+    for (MemberPlan& m : members_) m.rpt_cap = cap;
+}
+
+size_t FusedEngineImpl::passCount() const
+{
+    size_t n = 0;
+    for (const MemberPlan& m : members_) n += m.passes.size();
+    return n;
+}
+
 void FusedEngineImpl::setThreads(int threads, hipStream_t stream)
 {
     if (!threadsFit(threads)) fail("workgroup size must be 256 or 512 threads and fit every member");
@@ -755,6 +771,26 @@ float FusedEngineImpl::timeLaunches(const Schedule& s, hipStream_t stream) const
     enqueue(s, 0, 1, stream, nullptr);
     ETN_HIP(hipEventRecord(a, stream));
     enqueue(s, 1, std::min(n, 1 + kTimed), stream, nullptr);
+    ETN_HIP(hipEventRecord(b, stream));
+    ETN_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    ETN_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+// Every launch of a schedule, after one untimed launch.
+float FusedEngineImpl::timeSchedule(const Schedule& s, hipStream_t stream) const
+{
+    const size_t n = s.launches.size();
+    if (n < 1) return -1.0f;
+    hipEvent_t a, b;
+    ETN_HIP(hipEventCreate(&a));
+    ETN_HIP(hipEventCreate(&b));
+    enqueue(s, 0, 1, stream, nullptr);
+    ETN_HIP(hipEventRecord(a, stream));
+    enqueue(s, 0, n, stream, nullptr);
     ETN_HIP(hipEventRecord(b, stream));
     ETN_HIP(hipEventSynchronize(b));
     float ms = 0.0f;
@@ -801,12 +837,36 @@ void FusedEngineImpl::tuneTiles(hipStream_t stream)
 // the engine's whole schedule; keep the fastest shape with its tiles.
 void FusedEngineImpl::autotune(hipStream_t stream)
 {
-    std::vector<int> shapes;
-    for (int t : {512, 256})
-        if (threadsFit(t)) shapes.push_back(t);
-    bool any = shapes.size() > 1;
+    // Candidates: workgroup shape x cap on the rays per thread.  A lower cap
+    // needs fewer registers per thread (more waves per SIMD: two rays take
+    // about 120 VGPRs, eight about 190) but splits a block's forward
+    // projections over more passes, i.e. more sweeps of the volume; a cap
+    // that leaves the passes unchanged is not a candidate of its own.
+    struct Candidate {
+        int threads, cap;
+    };
+    std::vector<Candidate> cands;
+    for (int t : {512, 256}) {
+        if (!threadsFit(t)) continue;
+        useThreads(t);
+        size_t passes_at_max = 0;
+        for (int c : {dev::kMaxRaysPerThread, 4, 2}) {
+            useRptCap(c);
+            for (MemberPlan& m : members_) {
+                m.tile = default_tile(m);
+                build_passes(m);
+            }
+            const size_t n = passCount();
+            if (c == dev::kMaxRaysPerThread) passes_at_max = n;
+            else if (n == passes_at_max) continue;
+            cands.push_back({t, c});
+        }
+    }
+    bool any = cands.size() > 1;
     for (const MemberPlan& m : members_) any = any || candidate_tiles(m).size() > 1;
     if (!any) {
+        useThreads(cands.empty() ? threads_ : cands[0].threads);
+        useRptCap(cands.empty() ? dev::kMaxRaysPerThread : cands[0].cap);
         upload(stream);
         return;
     }
@@ -824,23 +884,27 @@ void FusedEngineImpl::autotune(hipStream_t stream)
                                hipMemcpyDeviceToDevice, stream));
         at += bytes[i];
     }
-    int best_threads = threads_;
+    // the candidates split an iteration into different passes, so each is
+    // timed over a whole iteration
+    Candidate best = cands[0];
     std::vector<Tile> best_tiles;
     float best_ms = -1.0f;
-    for (int t : shapes) {
-        useThreads(t);
+    for (const Candidate& c : cands) {
+        useThreads(c.threads);
+        useRptCap(c.cap);
         for (MemberPlan& m : members_) m.tile = default_tile(m);
         tuneTiles(stream);
         upload(stream);
-        const float ms = shapes.size() > 1 ? timeLaunches(sched_, stream) : 0.0f;
+        const float ms = cands.size() > 1 ? timeSchedule(sched_, stream) : 0.0f;
         if (best_tiles.empty() || (ms >= 0.0f && ms < best_ms)) {
             best_ms = ms;
-            best_threads = t;
+            best = c;
             best_tiles.clear();
             for (const MemberPlan& m : members_) best_tiles.push_back(m.tile);
         }
     }
-    useThreads(best_threads);
+    useThreads(best.threads);
+    useRptCap(best.cap);
     for (size_t i = 0; i < members_.size(); ++i) members_[i].tile = best_tiles[i];
     at = 0;
     for (size_t i = 0; i < members_.size(); ++i) {
