@@ -266,20 +266,36 @@ def support_mask(nx, ny, nz, dev):
 
 
 def run_precision(args):
-    """SART with f32, bf16 and f16 voxel storage on the same input: GUPS of
-    each and, after every iteration, the relative L2 distance of the 16-bit
-    volumes from the f32 volume (all FP32 arithmetic), plus the FP64 residual
-    norm of each."""
+    """Precision variant: SART on 1024x1024x256 x 121 tilts with the volume
+    stored in f32, bf16 and f16 (16-bit forms rounded to nearest even, and with
+    the opt-in stochastic rounding), FP32 ray accumulation throughout.
+
+    Full volume: GUPS of each form and, after every iteration, the relative L2
+    distance of each 16-bit volume from the f32 GPU volume.
+
+    Against the FP64 oracle: in this geometry volume plane y is traced only by
+    detector row y and back-projected only from it, so the planes are
+    independent up to the float rounding of the projection matrix (a plane's
+    detector-row weight is 0 only to rounding, so neighbouring planes couple
+    weakly), and a y-slab of the volume with its detector rows is one shard's
+    reconstruction.  The 16 planes y = 504..519 run as their own 1024x16x256
+    problem (the same coordinates: the slab straddles y = 0) on the GPU in every
+    form and in the FP64 oracle; after every iteration each GPU slab's relative
+    L2 distance from the FP64 slab is reported, and at the end the f32 slab's
+    distance from those planes of the full f32 volume."""
     import torch
     import ettention_amd as E
     from ettention_amd import api as A
     from ettention_amd import synthetic as S
+    from oracle import oracle as O
 
     wl = WORKLOADS["c3"]
     nx, ny, nz = wl["res"]
     nu, nv = wl["det"]
+    iters = wl["iterations"]
     tilts = S.tilt_series(wl["max_angle"], wl["step"])
     N = len(tilts)
+    V, P = nx * ny * nz, nu * nv
     vg = E.VolumeGeometry.make((nx, ny, nz))
     base = E.ScannerGeometry.mrc_base(nu, nv)
     dev = torch.device("cuda:0")
@@ -290,47 +306,94 @@ def run_precision(args):
         scratch.zero_()
         A.forward_project(vg, phantom, A.rotate(base, float(t)), projs[i], scratch)
     projs = S.add_gaussian_noise(projs, 0.02, seed=7)
-    del scratch
-    ops = {vt: A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
-                                                      projections=projs, algorithm="sart", lam=wl["lam"],
-                                                      projection_iteration=wl["order"], voxel_type=vt)
-           for vt in ("float", "bfloat16", "half")}
-    per_iter = {vt: [] for vt in ops}
-    elapsed = {vt: 0.0 for vt in ops}
-    for vt, r in ops.items():      # tile autotuning and graph capture outside the timed region
+    del scratch, phantom
+    torch_type = {"float": torch.float32, "bfloat16": torch.bfloat16, "half": torch.float16}
+    forms = {"float": ("float", 0), "bfloat16": ("bfloat16", 0), "half": ("half", 0),
+             "bfloat16_sr": ("bfloat16", E.ETN_RECON_FLAG_STOCHASTIC_ROUNDING),
+             "half_sr": ("half", E.ETN_RECON_FLAG_STOCHASTIC_ROUNDING)}
+
+    def make(volume, base_g, p):
+        return {f: A.BlockIterativeReconstructionOperator(volume=volume, base_geometry=base_g, tilt_angles=tilts,
+                                                          projections=p, algorithm="sart", lam=wl["lam"],
+                                                          projection_iteration=wl["order"], voxel_type=vt, flags=fl)
+                for f, (vt, fl) in forms.items()}
+
+    ops = make(vg, base, projs)
+    for f, r in ops.items():      # tile autotuning and graph capture outside the timed region
         r.run_iterations(1)
-        r.set_volume(torch.zeros(r.shape, dtype={"float": torch.float32, "bfloat16": torch.bfloat16,
-                                                 "half": torch.float16}[vt], device=dev))
-    for it in range(1, wl["iterations"] + 1):
+        r.set_volume(torch.zeros(r.shape, dtype=torch_type[forms[f][0]], device=dev))
+    elapsed = {f: 0.0 for f in ops}
+    per_iter = []
+    for it in range(1, iters + 1):
         vols = {}
-        for vt, r in ops.items():
+        for f, r in ops.items():
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             r.run_iterations(1)
             r.synchronize()
-            elapsed[vt] += time.perf_counter() - t0
-            vols[vt] = r.get_reconstructed_volume(dtype=torch.float32)
+            elapsed[f] += time.perf_counter() - t0
+            vols[f] = r.get_reconstructed_volume(dtype=torch.float32)
         ref = vols["float"]
         rn = float(torch.linalg.vector_norm(ref.double()))
         row = {"iteration": it}
-        for vt in ("bfloat16", "half"):
-            row[vt + "_rel_l2_vs_f32"] = float(torch.linalg.vector_norm((vols[vt] - ref).double())) / rn
-        per_iter["float"].append(row)
-        log("iteration %d: rel L2 vs f32  bf16 %.3e  f16 %.3e" % (it, row["bfloat16_rel_l2_vs_f32"],
-                                                                 row["half_rel_l2_vs_f32"]))
+        for f in forms:
+            if f != "float":
+                row[f] = float(torch.linalg.vector_norm((vols[f] - ref).double())) / rn
+        per_iter.append(row)
+        log("iteration %d: rel L2 vs f32 " % it + "  ".join("%s %.3e" % (f, row[f]) for f in forms if f != "float"))
         del vols, ref
-    norms = {vt: r.residual_norm() for vt, r in ops.items()}
-    V = nx * ny * nz
-    gups = {vt: V * N * wl["iterations"] / elapsed[vt] / 1e9 for vt in ops}
+    full_f32_slab = ops["float"].get_reconstructed_volume(dtype=torch.float32)[:, 504:520, :].cpu().numpy()
+    gups = {f: V * N * iters / elapsed[f] / 1e9 for f in ops}
     log("GUPS: " + ", ".join("%s %.1f" % kv for kv in gups.items()))
+    ms_bf16 = 1e3 * elapsed["bfloat16"] / iters
+    roofline = kernel_roofline(ops["bfloat16"], "bfloat16", True, V, P, 1, pmc=False, gpu_ms_per_step=ms_bf16, N=N)
+    roofline["kernel"] += ", bf16 storage"
+    roofline["time_source"] = "host clock around each synchronised iteration"
+    norms = {f: r.residual_norm() for f, r in ops.items()}
+    del ops
+    torch.cuda.empty_cache()
+
+    # the y-slab against the FP64 oracle
+    y0, sh = 504, 16
+    svg = E.VolumeGeometry.make((nx, sh, nz))
+    sbase = E.ScannerGeometry.mrc_base(nu, sh)
+    sprojs = projs[:, y0:y0 + sh, :].contiguous()
+    sops = make(svg, sbase, sprojs)
+    ov = O.Volume.make((nx, sh, nz))
+    spec = O.ReconSpec(ov, O.mrc_base_scanner(nu, sh), tilts, sprojs.cpu().numpy(), block_size=1, order=wl["order"],
+                       lam=wl["lam"])
+    d64 = np.zeros((nz, sh, nx), np.float64)
+    vs_oracle = []
+    t_oracle = 0.0
+    for it in range(1, iters + 1):
+        t0 = time.perf_counter()
+        spec.run_f64(d64, 1)
+        t_oracle += time.perf_counter() - t0
+        dn = np.linalg.norm(d64)
+        row = {"iteration": it}
+        for f, r in sops.items():
+            r.run_iterations(1)
+            got = r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy().astype(np.float64)
+            row[f] = float(np.linalg.norm(got - d64) / dn)
+        vs_oracle.append(row)
+        log("slab iteration %d: rel L2 vs FP64 oracle " % it + "  ".join("%s %.3e" % (f, row[f]) for f in forms))
+    slab_f32 = sops["float"].get_reconstructed_volume(dtype=torch.float32).cpu().numpy()
+    slab_vs_full = float(np.linalg.norm((slab_f32 - full_f32_slab).astype(np.float64)) /
+                         np.linalg.norm(full_f32_slab.astype(np.float64)))
+    log("FP64 oracle: %d slab iterations in %.1f s; f32 slab vs the full volume's planes: rel L2 %.3e"
+        % (iters, t_oracle, slab_vs_full))
+    del sops
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, tilts, projs, 6)
     out = {
         "metric": "SART GUPS, bf16 storage, 1024x1024x256 x 121 tilts (precision variant)",
         "value": round(gups["bfloat16"], 3),
         "unit": "GUPS",
         "n_gpus": 1,
-        "steps": wl["iterations"],
+        "steps": iters,
         "warmup": 1,
-        "ms_per_step": round(1e3 * elapsed["bfloat16"] / wl["iterations"], 4),
+        "ms_per_step": round(ms_bf16, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -338,11 +401,14 @@ def run_precision(args):
         "data": "synthetic (seeded ellipsoid phantom, FP projections + 2% Gaussian noise)",
         "config": {"workload": wl["name"], "volume": [nx, ny, nz], "detector": [nu, nv], "projections": N,
                    "parallelism": "single GPU"},
-        "gups": {vt: round(g, 3) for vt, g in gups.items()},
+        "gups": {f: round(g, 3) for f, g in gups.items()},
         "residual_norm_after": norms,
-        "rel_l2_vs_f32_per_iteration": per_iter["float"],
-        "roofline": None,
-        "cpu_baseline": None,
+        "rel_l2_vs_f32_per_iteration": per_iter,
+        "rel_l2_vs_fp64_oracle_per_iteration": vs_oracle,
+        "fp64_slab": {"planes": [y0, y0 + sh], "volume": [nx, sh, nz], "detector": [nu, sh],
+                      "f32_slab_vs_full_volume_planes_rel_l2": slab_vs_full, "oracle_seconds": round(t_oracle, 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -27,6 +27,7 @@ ETN_RECON_FLAG_GENERIC_KERNELS = 1
 ETN_RECON_FLAG_NO_GRAPH = 2
 ETN_RECON_FLAG_NO_AUTOTUNE = 4
 ETN_RECON_FLAG_EXACT_ARITHMETIC = 8
+ETN_RECON_FLAG_STOCHASTIC_ROUNDING = 16
 
 
 class EtnError(RuntimeError):

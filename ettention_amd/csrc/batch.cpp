@@ -20,7 +20,8 @@ BatchReconstruction::BatchReconstruction(const etn_reconstruction_desc* members,
         std::vector<etn_reconstruction_desc> descs(members, members + count);
         for (auto& d : descs) {
             const etn_reconstruction_desc own = d;
-            d.flags = own.flags | ETN_RECON_FLAG_GENERIC_KERNELS | ETN_RECON_FLAG_NO_GRAPH | ETN_RECON_FLAG_NO_AUTOTUNE;
+            d.flags = (own.flags & ~(uint32_t)ETN_RECON_FLAG_STOCHASTIC_ROUNDING) | ETN_RECON_FLAG_GENERIC_KERNELS |
+                      ETN_RECON_FLAG_NO_GRAPH | ETN_RECON_FLAG_NO_AUTOTUNE;
             ops_.push_back(std::make_unique<BlockIterativeReconstructionOperator>(d, stream_));
         }
         slots_.assign(count, Slot{});
@@ -46,6 +47,10 @@ BatchReconstruction::BatchReconstruction(const etn_reconstruction_desc* members,
                 left.swap(rest);
             }
         }
+        for (uint32_t i = 0; i < count; ++i)
+            if ((members[i].flags & ETN_RECON_FLAG_STOCHASTIC_ROUNDING) && slots_[i].engine < 0)
+                fail("batch member " + std::to_string(i) + ": stochastic rounding needs the fast fused kernels "
+                     "(f16 or bf16 storage, no mask, no oversampling, nx a multiple of 4, nx + nz <= 2049)");
     } catch (...) {
         ops_.clear();
         engines_.clear();

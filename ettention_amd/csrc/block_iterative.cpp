@@ -95,6 +95,9 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
         fused_ = FusedPlaneEngine::create({{this, &d}}, (d.flags & ETN_RECON_FLAG_EXACT_ARITHMETIC) != 0, stream_,
                                           &taken);
     }
+    if ((d.flags & ETN_RECON_FLAG_STOCHASTIC_ROUNDING) && !fused_)
+        fail("stochastic rounding needs the fast fused kernels: f16 or bf16 storage, no mask, no oversampling, "
+             "single-axis tilt, nx a multiple of 4 and nx + nz <= 2049");
     tuned_ = (d.flags & ETN_RECON_FLAG_NO_AUTOTUNE) != 0;
 }
 

@@ -37,11 +37,30 @@ __device__ __forceinline__ ETN_GLOBAL T* global_ptr(T* p)
 
 // ---- voxel storage ---------------------------------------------------------
 // load/store take any pointer to T (generic or global)
+// Stochastic rounding (opt-in for 16-bit storage): f goes to one of the two
+// representable values enclosing it, "down" (toward zero) and "up", and to
+// "up" with probability (f - down) / (up - down), so the stored value is f in
+// expectation: updates smaller than half a unit in the last place survive on
+// average instead of being dropped.  r holds uniformly distributed bits.
+__device__ __forceinline__ uint32_t sr_hash(uint32_t index, uint32_t salt, float value)
+{
+    // murmur3's 32-bit finaliser over the voxel index, the projection's salt
+    // and the value being rounded (which changes from one update to the next)
+    uint32_t h = index * 0x9E3779B1u ^ salt * 0x85EBCA77u ^ __float_as_uint(value);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
 struct F32 {
     using T = float;
     template <typename P>
     static __device__ __forceinline__ float load(P p, size_t i) { return p[i]; }
     static __device__ __forceinline__ float round(float f) { return f; }
+    static __device__ __forceinline__ float round_stochastic(float f, uint32_t) { return f; }
     template <typename P>
     static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = f; }
 };
@@ -54,6 +73,18 @@ struct F16 {   // vload_half / vstore_half (round to nearest even), HalfFloatSup
         return __half2float(h);
     }
     static __device__ __forceinline__ float round(float f) { return __half2float(__float2half_rn(f)); }
+    static __device__ __forceinline__ float round_stochastic(float f, uint32_t r)
+    {
+        // down: f rounded toward zero (exact when f is representable); up: the
+        // next half away from zero (the sign kept; past the largest finite
+        // half that is infinity, as round to nearest gives there too)
+        const __half dn = __float2half_rz(f);
+        const float d = __half2float(dn);
+        const __half up = __ushort_as_half((unsigned short)(__half_as_ushort(dn) + 1u));
+        const float u = __half2float(up);
+        const float t = (float)(r >> 8) * 0x1.0p-24f;   // uniform in [0, 1)
+        return fabsf(f - d) > t * fabsf(u - d) ? u : d;
+    }
     template <typename P>
     static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = __float2half_rn(f); }
 };
@@ -73,6 +104,14 @@ struct BF16 {  // bfloat16, round to nearest even
     template <typename P>
     static __device__ __forceinline__ float load(P p, size_t i) { return unpack(p[i]); }
     static __device__ __forceinline__ float round(float f) { return unpack(pack(f)); }
+    static __device__ __forceinline__ float round_stochastic(float f, uint32_t r)
+    {
+        // add 16 random bits below the kept mantissa and truncate: the
+        // magnitude rounds up with probability equal to the dropped fraction
+        const uint32_t u = __float_as_uint(f);
+        if ((u & 0x7fffffffu) >= 0x7f800000u) return f;   // infinity, NaN
+        return __uint_as_float((u + (r & 0xffffu)) & 0xffff0000u);
+    }
     template <typename P>
     static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = pack(f); }
 };

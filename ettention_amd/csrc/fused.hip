@@ -60,6 +60,7 @@ struct MemberPlan {
     int threads = dev::kMaxThreads;    // workgroup shape of the engine's launches
     int vbytes = 4;                    // bytes of a stored voxel (the 512-thread shape's LDS array depends on it)
     int rpt_cap = dev::kMaxRaysPerThread;  // rays per thread a pass may give its forward projections
+    bool sr = false;                   // stochastic rounding of the back projections' updates to 16-bit storage
     int fp_n = 1, fp_shift = 0;        // forward sub-rays per pixel axis; log2 of ray slots (columns si) per pixel
     int bp_samples = 1, rowsets = 1;   // back projection samples per voxel; pattern rows
     bool oversampled() const { return fp_n != 1 || bp_samples != 1; }
@@ -386,6 +387,7 @@ void build_passes(MemberPlan& mb)
             e.sample = make_float4(smp[0].dx, smp[0].dy, smp[0].dz, smp[0].length);
             e.nsamples = (int)smp.size();
             e.per_dir = mb.rowsets;
+            e.sr_salt = mb.sr ? op.getPhysicalProjectionIndex(b, p) + 1u : 0u;
             for (size_t k = 0; k < smp.size(); ++k)
                 e.samples[k] = make_float4(smp[k].dx, smp[k].dy, smp[k].dz, smp[k].length);
             pending.push_back(e);
@@ -438,6 +440,7 @@ private:
     bool vec_, cap_, exact_;
     bool mask_;                         // every member has a mask (the launches use the mask kernels)
     bool over_;                         // every member oversamples (the launches use the oversampled kernels)
+    bool sr_;                           // stochastic rounding to 16-bit storage
     int threads_ = dev::kMaxThreads;    // workgroup shape of every launch
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
@@ -446,7 +449,8 @@ private:
 
 FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
                                  bool mask, bool over, hipStream_t stream)
-    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact), mask_(mask), over_(over)
+    : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact), mask_(mask), over_(over),
+      sr_(!members_.empty() && members_[0].sr)
 {
     diag_.allocate(sizeof(int));
     ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
@@ -592,7 +596,7 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         const Launch& l = s.launches[k];
         if (l.groups == 0) continue;
         const dev::FusedLaunch fl{threads_, l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
-                                  tables + k * members_.size(), l.lds};
+                                  tables + k * members_.size(), l.lds, sr_};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
         dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
@@ -1064,20 +1068,28 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
     taken->assign(members.size(), false);
     std::vector<MemberPlan> plans;
     etn_voxel_type vt = ETN_VOXEL_FLOAT32;
-    bool vec = true, cap = false, mask = false, over = false;
+    bool vec = true, cap = false, mask = false, over = false, sr = false;
     for (size_t i = 0; i < members.size(); ++i) {
         const etn_reconstruction_desc& d = *members[i].desc;
         const bool d_over = (d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1;
-        if (!plans.empty() && (d.voxel_type != vt || (d.mask != nullptr) != mask || d_over != over ||
+        const bool d_sr = (d.flags & ETN_RECON_FLAG_STOCHASTIC_ROUNDING) != 0;
+        if (!plans.empty() && (d.voxel_type != vt || (d.mask != nullptr) != mask || d_over != over || d_sr != sr ||
                                (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
+            continue;
+        // stochastic rounding exists on the fast 16-bit kernels without mask or oversampling
+        if (d_sr && (exact || d.voxel_type == ETN_VOXEL_FLOAT32 || d.mask != nullptr || d_over ||
+                     members[i].op->getReconstructedVolume()->properties().nx() % 4 != 0))
             continue;
         MemberPlan m;
         if (!plan_member(*members[i].op, d, exact, m)) continue;
+        if (d_sr && m.nx + m.nz > 2049) continue;
+        m.sr = d_sr;
         if (plans.empty()) {
             vt = d.voxel_type;
             vec = m.nx % 4 == 0;
             mask = m.mask != nullptr;
             over = m.oversampled();
+            sr = m.sr;
         }
         cap = cap || m.nx + m.nz > 2049;
         plans.push_back(std::move(m));

@@ -97,6 +97,7 @@ struct FusedBp {
     // per_dir samples of one pattern row share a y offset)
     int nsamples, per_dir;
     float4 samples[kMaxFusedBpSamples];
+    uint32_t sr_salt;            // stochastic-rounding kernels: per-projection salt of the random bits
 };
 
 struct FusedPass {
@@ -154,6 +155,7 @@ struct FusedLaunch {
     const FusedWork* work;
     const FusedPass* const* passes;
     size_t lds;
+    bool sr;                     // stochastic rounding to 16-bit storage (fast kernels, no mask or oversampling)
 };
 
 namespace {
@@ -361,7 +363,7 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // and adds its sums fp_n times, in the generic kernel's (si, sj) order.
 // With at most two rays per thread the kernel fits 128 VGPRs: four waves per
 // SIMD, two 512-thread workgroups per CU (as LDS also allows).
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT, bool kSR>
 __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
@@ -628,6 +630,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
             const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda;
             const float4 smp = bp.sample;
             const float chord = smp.w, lc = lam * chord;
+            const uint32_t salt = kSR ? bp.sr_salt : 0u;
             // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
             auto x_term = [&](int x, float a4) {
                 const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + smp.x;
@@ -638,7 +641,8 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                 return m8 * pz;
             };
             // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
-            auto update = [&](const float4* pr, float ty, float xt, float zt, float value, float mv) {
+            // vid: the voxel's index in the volume (stochastic rounding's random bits)
+            auto update = [&](const float4* pr, float ty, float xt, float zt, float value, float mv, uint32_t vid) {
                 if (kFast) {
                     // u in the reference's order (u is a few hundred pixels, so a
                     // reordered sum would move the weights by ~1e-4); fract without
@@ -657,6 +661,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                     const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
                     const float next = kMask ? fmaf(mv, contrib, value) : value + contrib;
                     value = (L > kEpsilon) ? next : value;
+                    if constexpr (kSR) return S::round_stochastic(value, sr_hash(vid, salt, value));
                     return S::round(value);
                 }
                 float u = xt + zt;
@@ -758,20 +763,22 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                     const float zt = z_term(zz);
                     float out[G];
                     float* cell = band + zz * width + xw;
+                    const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
                     if constexpr (G == 4) {
                         // four voxels, 16-byte aligned (width and xw are multiples of 4)
                         const float4 in = *reinterpret_cast<const float4*>(cell);
                         const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskOff)
                                                 : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-                        out[0] = update(pr, pt.y, xt[0], zt, in.x, mk.x);
-                        out[1] = update(pr, pt.y, xt[1], zt, in.y, mk.y);
-                        out[2] = update(pr, pt.y, xt[2], zt, in.z, mk.z);
-                        out[3] = update(pr, pt.y, xt[3], zt, in.w, mk.w);
+                        out[0] = update(pr, pt.y, xt[0], zt, in.x, mk.x, vid);
+                        out[1] = update(pr, pt.y, xt[1], zt, in.y, mk.y, vid + 1u);
+                        out[2] = update(pr, pt.y, xt[2], zt, in.z, mk.z, vid + 2u);
+                        out[3] = update(pr, pt.y, xt[3], zt, in.w, mk.w, vid + 3u);
                         *reinterpret_cast<float4*>(cell) = make_float4(out[0], out[1], out[2], out[3]);
                     } else {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskOff] : 1.0f);
+                            out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskOff] : 1.0f,
+                                            vid + (uint32_t)i);
                             cell[i] = out[i];
                         }
                     }
@@ -791,10 +798,12 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                     const float4* pr = recs + p * kRecStride * row_len;
                     const float zt = z_term(zz);
                     float out[G];
+                    const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int c = zz * width + xw + i;
-                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskOff] : 1.0f);
+                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskOff] : 1.0f,
+                                        vid + (uint32_t)i);
                         band[c] = out[i];
                     }
                     if (q == nbp - 1)
@@ -890,17 +899,17 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         }
 }
 
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, bool kSR = false>
 void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
 {
     if (l.lds > sizeof(float) * (size_t)FusedShape<kThreads, (int)sizeof(typename S::T)>::kLds)
         fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kThreads);
     switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
     }
 }
 
@@ -924,6 +933,19 @@ void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 template <typename S, bool kFast, bool kMask, bool kOver>
 void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
+    if (l.sr) {
+        // stochastic rounding: 16-bit storage on the fast kernels without a
+        // mask, oversampling or the step limit, nx a multiple of 4
+        if constexpr (kFast && !kMask && !kOver && !std::is_same<S, F32>::value) {
+            if (vec && !cap) {
+                if (l.threads == 512) launch_fused_shape<S, true, false, true, false, false, 512, true>(l, stream);
+                else launch_fused_shape<S, true, false, true, false, false, 256, true>(l, stream);
+                return;
+            }
+        }
+        fail("stochastic rounding needs 16-bit storage, fast arithmetic, no mask, no oversampling, nx a multiple "
+             "of 4 and nx + nz <= 2049");
+    }
     if (vec && !cap) launch_fused_rpt<S, true, false, kFast, kMask, kOver>(l, stream);
     else if (vec) launch_fused_rpt<S, true, true, kFast, kMask, kOver>(l, stream);
     else if (!cap) launch_fused_rpt<S, false, false, kFast, kMask, kOver>(l, stream);

@@ -185,7 +185,15 @@ enum {
     ETN_RECON_FLAG_EXACT_ARITHMETIC = 1u << 3   /* fused kernels repeat the generic kernels' float operations in
                                                    their order (bit-identical volumes); by default they combine
                                                    the bilinear weights per detector column, use the hardware
-                                                   reciprocal for chord / L and fused multiply-adds */
+                                                   reciprocal for chord / L and fused multiply-adds */,
+    ETN_RECON_FLAG_STOCHASTIC_ROUNDING = 1u << 4 /* 16-bit storage: round each back-projected voxel to storage
+                                                   stochastically (up with probability equal to the dropped
+                                                   fraction; random bits hashed from the voxel index, the
+                                                   projection and the value, so runs are reproducible) instead
+                                                   of to nearest even.  Extension (the reference rounds to
+                                                   nearest); needs the fast fused kernels: f16 or bf16 storage,
+                                                   no mask, no oversampling, nx a multiple of 4, nx + nz <= 2049;
+                                                   creation fails otherwise */
 };
 
 /* Parameters of BlockIterativeReconstructionOperator, named after the
