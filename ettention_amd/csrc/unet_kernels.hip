@@ -123,6 +123,147 @@ __global__ void __launch_bounds__(kThreads) conv3_fwd_kernel(ConvArgs a)
         }
 }
 
+// ---- 3x3x3 convolution to 16 or 32 channels, LDS-staged bricks ----------
+// A workgroup computes a kBZ x kBY x kBX brick of output voxels, all
+// NT x 16 output channels.  Its input halo (kBZ + 2) x (kBY + 2) x (kBX + 2)
+// is staged in LDS eight input channels at a time (one 16-byte cell per halo
+// voxel), so that the 27 taps read each staged voxel from LDS instead of from
+// memory.  An MFMA k step takes four taps of the chunk: lane group kg's A
+// operand is its voxel's 8 channels at tap 4 s + kg, and its B operand the 8
+// weights wt[co][tap * cin + ci ..] of the same tap, contiguous in the forward
+// weight layout (the k order inside a step only has to agree between A and
+// B).  The next chunk's weights and halo are loaded into registers while the
+// current chunk's MFMAs run.  Wave w owns the brick's output rows
+// (z = w, y = 0..7), two 16-voxel tiles each: the LDS reads of a step share one
+// base address and differ by constant offsets.  The result goes through LDS
+// (one 16-channel slice at a time) so that each voxel's channels leave in
+// 16-byte stores.
+constexpr int kBZ = kConvWaves, kBY = 8, kBX = 32;
+constexpr int kHZ = kBZ + 2, kHY = kBY + 2, kHX = kBX + 2;
+constexpr int kHaloCells = kHZ * kHY * kHX;                    // 2040
+constexpr int kBrickVoxels = kBZ * kBY * kBX;                  // 1024
+constexpr int kBrickTiles = kBrickVoxels / 16 / kConvWaves;    // 16 tiles of 16 voxels per wave
+constexpr int kStage = (kHaloCells + kThreads - 1) / kThreads; // halo cells per thread
+constexpr int kChunkSteps = (27 + 3) / 4;                      // k steps per 8-channel chunk
+constexpr int kBrickLds = 2 * kBrickVoxels;                    // cells: >= the halo, = one output slice
+
+template <int NT>
+__global__ void __launch_bounds__(kThreads) conv3_brick_kernel(ConvArgs a, int nbx, int nby)
+{
+    static_assert(kBrickLds >= kHaloCells, "the output slice reuses the halo's LDS");
+    __shared__ u32x4 lds[kBrickLds];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int bxi = blockIdx.x % nbx, rest = blockIdx.x / nbx;
+    const int byi = rest % nby, bzi = rest / nby;
+    const int x0 = bxi * kBX, y0 = byi * kBY, z0 = bzi * kBZ;
+    const int ca = a.src.ca, cin = a.src.ca + a.src.cb;
+    const int Hh = a.H >> 1, Wh = a.W >> 1;
+    const u32x4 zero4 = u32x4{0u, 0u, 0u, 0u};
+
+    floatx4 acc[kBrickTiles][NT];
+#pragma unroll
+    for (int t = 0; t < kBrickTiles; ++t)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[t][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // one chunk's weights (step st, tap 4 st + kg, output channel nt * 16 + row)
+    // and halo cells, loaded into registers; the next chunk's loads are in
+    // flight while the current chunk's MFMAs run
+    u32x4 B[kChunkSteps][NT], cell[kStage];
+    auto load_chunk = [&](int c0) {
+#pragma unroll
+        for (int st = 0; st < kChunkSteps; ++st) {
+            const int tap = 4 * st + kg;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                B[st][nt] = tap < 27 ? load16(a.wt + (size_t)(nt * 16 + row) * a.kpad + tap * cin + c0) : zero4;
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int e = k * kThreads + tid;
+            cell[k] = zero4;
+            if (e < kHaloCells) {
+                const int hz = e / (kHY * kHX), r2 = e - hz * (kHY * kHX), hy = r2 / kHX, hx = r2 - hy * kHX;
+                const int zz = z0 - 1 + hz, yy = y0 - 1 + hy, xx = x0 - 1 + hx;
+                if ((unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W) {
+                    const uint16_t* p;
+                    if (c0 < ca) {
+                        const size_t vi = a.src.a_half ? ((size_t)(zz >> 1) * Hh + (yy >> 1)) * Wh + (xx >> 1)
+                                                       : ((size_t)zz * a.H + yy) * a.W + xx;
+                        p = a.src.a + vi * ca + c0;
+                    } else {
+                        p = a.src.b + (((size_t)zz * a.H + yy) * a.W + xx) * a.src.cb + (c0 - ca);
+                    }
+                    cell[k] = load16(p);
+                }
+            }
+        }
+    };
+    load_chunk(0);
+    for (int c0 = 0; c0 < cin; c0 += 8) {
+        __syncthreads();   // the previous chunk's reads of lds are done
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int e = k * kThreads + tid;
+            if (e < kHaloCells) lds[e] = cell[k];
+        }
+        u32x4 Bc[kChunkSteps][NT];
+#pragma unroll
+        for (int st = 0; st < kChunkSteps; ++st)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) Bc[st][nt] = B[st][nt];
+        __syncthreads();
+        if (c0 + 8 < cin) load_chunk(c0 + 8);
+#pragma unroll
+        for (int st = 0; st < kChunkSteps; ++st) {
+            const int tap = 4 * st + kg;
+            const bool live = tap < 27;
+            const int tc = live ? tap : 0;   // a valid address for the padding taps (B is zero there)
+            const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;
+            const u32x4* base = lds + ((wave + dz) * kHY + dy) * kHX + dx + row;
+#pragma unroll
+            for (int t = 0; t < kBrickTiles; ++t) {
+                u32x4 av = base[(t >> 1) * kHX + (t & 1) * 16];
+                if (!live) av = zero4;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av), as_bf16x8(Bc[st][nt]),
+                                                                         acc[t][nt], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue, one 16-channel slice at a time: bias, activation, bf16 into
+    // LDS as [brick voxel][16 channels], then 16-byte stores of 8 channels
+    uint16_t* slice = reinterpret_cast<uint16_t*>(lds);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int co = nt * 16 + row;
+        const float b = a.bias ? a.bias[co] : 0.0f;
+        __syncthreads();   // the last chunk's reads (or the previous slice's copy) are done
+#pragma unroll
+        for (int t = 0; t < kBrickTiles; ++t) {
+            const int v0 = (wave * kBY + (t >> 1)) * kBX + (t & 1) * 16 + kg * 4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v = acc[t][nt][e] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                slice[(v0 + e) * 16 + row] = f2bf(v);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kBrickLds / kThreads; ++k) {
+            const int c = k * kThreads + tid;         // cell: voxel c / 2, channels 8 (c % 2) ..
+            const int bv = c >> 1, bz = bv / (kBY * kBX), by = (bv / kBX) % kBY, bx = bv % kBX;
+            const int z = z0 + bz, y = y0 + by, x = x0 + bx;
+            if (z < a.D && y < a.H && x < a.W)
+                *reinterpret_cast<u32x4*>(a.out + (((size_t)z * a.H + y) * a.W + x) * a.cout + nt * 16 + (c & 1) * 8) =
+                    lds[c];
+        }
+    }
+}
+
 template <int NT, int MT>
 void launch_conv(const ConvArgs& a, hipStream_t stream)
 {
@@ -466,9 +607,11 @@ void conv3_forward(const ConvArgs& a, hipStream_t stream)
     if (a.src.ca < 8 || (a.src.ca % 8) || (a.src.cb % 8)) fail("conv3: input channels must be multiples of 8");
     if (a.kpad != kpad_for(a.src.ca + a.src.cb)) fail("conv3: weight row length mismatch");
     if (a.src.a_half && ((a.D | a.H | a.W) & 1)) fail("conv3: upsampled input needs even sizes");
+    const int nbx = (a.W + kBX - 1) / kBX, nby = (a.H + kBY - 1) / kBY, nbz = (a.D + kBZ - 1) / kBZ;
+    const unsigned bricks = (unsigned)(nbx * nby * nbz);
     switch (a.cout) {
-        case 16: launch_conv<1, 4>(a, stream); break;
-        case 32: launch_conv<2, 4>(a, stream); break;
+        case 16: conv3_brick_kernel<1><<<bricks, kThreads, 0, stream>>>(a, nbx, nby); break;
+        case 32: conv3_brick_kernel<2><<<bricks, kThreads, 0, stream>>>(a, nbx, nby); break;
         case 48: launch_conv<3, 2>(a, stream); break;
         case 64: launch_conv<4, 2>(a, stream); break;
         case 96: launch_conv<6, 2>(a, stream); break;
