@@ -254,6 +254,8 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
     lrn.run_iterations(1)
     lrn.set_volume(torch.zeros(lrn.shape, dtype=torch.bfloat16, device="cuda"))
 
+    jumped = []
+
     def jump():
         lrn.run_iterations(k0 - 1)
         prev = lrn.get_reconstructed_volume(dtype=torch.float32)
@@ -261,18 +263,37 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
         x = lrn.get_reconstructed_volume(dtype=torch.float32)
         y = net.predict(x, x - prev, scale)
         lrn.set_volume(y.to(torch.bfloat16))
+        jumped.append(y)
 
     t_jump = timed(jump)
     jump_norm = lrn.residual_norm()
-    t_learned, norm, extra, best_norm, since_best = t_jump, jump_norm, 0, jump_norm, 0
-    while norm > target and k0 + extra < k_max and since_best < patience:
-        t_learned += timed(lambda: lrn.run_iterations(1))
-        extra += 1
-        norm = lrn.residual_norm()
-        if norm < best_norm:
-            best_norm, since_best = norm, 0
-        else:
-            since_best += 1
+
+    def polish(op, t0):
+        """Iterates op from the jumped volume until the target, k_max or patience."""
+        t, norm, extra, best_norm, since_best = t0, jump_norm, 0, jump_norm, 0
+        while norm > target and k0 + extra < k_max and since_best < patience:
+            t += timed(lambda: op.run_iterations(1))
+            extra += 1
+            norm = op.residual_norm()
+            if norm < best_norm:
+                best_norm, since_best = norm, 0
+            else:
+                since_best += 1
+        return t, norm, extra
+
+    t_learned, norm, extra = polish(lrn, t_jump)
+    # the same jump, then FP32 SART instead of bf16 (the conversion is timed)
+    f32 = A.BlockIterativeReconstructionOperator(algorithm="sart", lam=lam, projection_iteration=order, **case)
+    f32.run_iterations(1)
+    t_conv = timed(lambda: f32.set_volume(jumped[0]))
+    t_f32, norm_f32, extra_f32 = polish(f32, t_jump + t_conv)
+    del f32
+    # plain bf16 SART for reference (untimed): its own lowest residual
+    lrn.set_volume(torch.zeros(lrn.shape, dtype=torch.bfloat16, device="cuda"))
+    b_norms = []
+    for k in range(1, min(k_max, k_best + patience) + 1):
+        lrn.run_iterations(1)
+        b_norms.append(lrn.residual_norm())
     t_iter = sum(p_times) / len(p_times)
     k_equal = min(max(int(t_jump / t_iter), 1), len(p_norms))
     accepted = jump_norm < p_norms[k_equal - 1]
@@ -283,12 +304,20 @@ def time_to_target(case, net, scale, k_max=50, k0=3, lam=0.3, order="random", pa
            "plain_norm_at_jump_time": p_norms[k_equal - 1], "plain_iterations_at_jump_time": k_equal,
            "accepted": bool(accepted), "learned_extra_iterations": extra, "learned_s": t_learned,
            "learned_final_norm": norm, "reached": bool(reached),
-           "speedup": t_plain / t_learned if reached else None, "plain_norms": p_norms}
+           "speedup": t_plain / t_learned if reached else None, "plain_norms": p_norms,
+           "f32_polish_extra_iterations": extra_f32, "f32_polish_s": t_f32, "f32_polish_final_norm": norm_f32,
+           "f32_polish_reached": bool(norm_f32 <= target),
+           "f32_polish_speedup": t_plain / t_f32 if norm_f32 <= target else None,
+           "bf16_plain_best_norm": min(b_norms), "bf16_plain_best_iteration": 1 + b_norms.index(min(b_norms))}
     if log:
         log("plain FP32 SART: lowest residual %.6e at iteration %d, %.3f s" % (target, k_best, t_plain))
         log("bf16 SART + network: jump after %d iterations in %.3f s -> %.6e (plain at equal time, %d it: %.6e); "
             "%d more iterations, %.3f s total, target %s" % (k0, t_jump, jump_norm, k_equal, p_norms[k_equal - 1],
                                                            extra, t_learned, "reached" if reached else "missed"))
+        log("same jump, then FP32 SART: %d more iterations, %.3f s total, %.6e, target %s; plain bf16 SART's "
+            "lowest residual %.6e at iteration %d" % (extra_f32, t_f32, norm_f32,
+                                                     "reached" if norm_f32 <= target else "missed",
+                                                     min(b_norms), 1 + b_norms.index(min(b_norms))))
     return res
 
 
