@@ -28,6 +28,7 @@ ETN_RECON_FLAG_NO_GRAPH = 2
 ETN_RECON_FLAG_NO_AUTOTUNE = 4
 ETN_RECON_FLAG_EXACT_ARITHMETIC = 8
 ETN_RECON_FLAG_STOCHASTIC_ROUNDING = 16
+ETN_RECON_FLAG_16BIT_PROJECTIONS = 32
 
 
 class EtnError(RuntimeError):

@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (ETN_ALL_SUBVOLUMES, ETN_VOXEL_BFLOAT16, ETN_VOXEL_FLOAT16, ETN_VOXEL_FLOAT32,
+from ._lib import (ETN_RECON_FLAG_16BIT_PROJECTIONS, ETN_ALL_SUBVOLUMES, ETN_VOXEL_BFLOAT16, ETN_VOXEL_FLOAT16, ETN_VOXEL_FLOAT32,
                    ReconstructionDesc, ScannerGeometry, VolumeGeometry, call, lib)
 
 _VOXEL_TYPES = {"float": ETN_VOXEL_FLOAT32, "float32": ETN_VOXEL_FLOAT32, "half": ETN_VOXEL_FLOAT16,
@@ -221,9 +221,19 @@ def _make_desc(*, volume, base_geometry, tilt_angles, projections, algorithm="sa
     geometries: optional sequence of ScannerGeometry, one per projection, used
     instead of rotating base_geometry by the tilt angles."""
     tilts = np.ascontiguousarray(tilt_angles, np.float32)
-    # the library reads len(tilts) * nv * nu floats of projections and
-    # nx * ny * nz bytes of mask
-    _check(projections, "projections", ("float32",), (len(tilts),) + _image_shape(base_geometry))
+    # the library reads len(tilts) * nv * nu values of projections and
+    # nx * ny * nz bytes of mask.  16-bit projections (stored in the volume's
+    # 16-bit type) set ETN_RECON_FLAG_16BIT_PROJECTIONS.
+    torch = _torch()
+    vt = voxel_type_id(voxel_type)
+    sixteen = {ETN_VOXEL_FLOAT16: "float16", ETN_VOXEL_BFLOAT16: "bfloat16"}
+    if projections is not None and projections.dtype in (torch.float16, torch.bfloat16):
+        if vt not in sixteen:
+            raise ValueError("16-bit projections need voxel_type 'half' or 'bfloat16' (they take the volume's type)")
+        _check(projections, "projections", (sixteen[vt],), (len(tilts),) + _image_shape(base_geometry))
+        flags = int(flags) | ETN_RECON_FLAG_16BIT_PROJECTIONS
+    else:
+        _check(projections, "projections", ("float32",), (len(tilts),) + _image_shape(base_geometry))
     _check(mask, "mask", ("uint8",), _volume_shape(volume))
     skip = np.asarray(list(skip_projections) or [0], np.uint32)
     alg = algorithm.encode()

@@ -32,6 +32,12 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
     if (general_block_size_ == 0) fail("Illegal ProjectionBlockSize!");
     if (d.projection_count == 0 || (!d.tilt_angles && !d.geometries)) fail("No projections given!");
     if (!d.projections) fail("projection stack device pointer is null");
+    if (d.flags & ETN_RECON_FLAG_16BIT_PROJECTIONS) {
+        // the measured projections are stored in the volume's 16-bit type
+        if (d.voxel_type != ETN_VOXEL_FLOAT16 && d.voxel_type != ETN_VOXEL_BFLOAT16)
+            fail("16-bit projections need f16 or bf16 voxel storage (they take the volume's type)");
+        projection_type_ = d.voxel_type;
+    }
     if (d.base_geometry.resolution[0] == 0 || d.base_geometry.resolution[1] == 0) fail("detector resolution is zero");
     pixels_ = (size_t)d.base_geometry.resolution[0] * d.base_geometry.resolution[1];
     desc_.algorithm = nullptr;
@@ -87,7 +93,9 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
     for (uint32_t i = 0; i < d.projection_count; ++i)
         if (!skip.count(i)) measured_.push_back(i);
     for (uint32_t i = 0; i < d.projection_count; ++i)
-        real_.push_back({const_cast<float*>(realProjection(i)), d.base_geometry.resolution[0], d.base_geometry.resolution[1]});
+        // (a 16-bit projection's address; the compare kernel reads it in projectionType())
+        real_.push_back({static_cast<float*>(const_cast<void*>(realProjection(i))), d.base_geometry.resolution[0],
+                         d.base_geometry.resolution[1]});
     norm_scratch_.allocate(sizeof(double) * (dev::kSsdBlocks + 1));
 
     if ((d.flags & ETN_RECON_FLAG_GENERIC_KERNELS) == 0) {
@@ -251,7 +259,7 @@ void BlockIterativeReconstructionOperator::computeResidualsForBlock(uint32_t blo
             loc_->setTraversalLength(&traversal_[p]);
             loc_->run();
         }
-        compare_->setInput(&real_[getPhysicalProjectionIndex(block, p)], &virtual_[p]);
+        compare_->setInput(&real_[getPhysicalProjectionIndex(block, p)], &virtual_[p], projection_type_);
         compare_->setOutput(&residual_[p]);
         compare_->run();
     }
@@ -289,7 +297,8 @@ double BlockIterativeReconstructionOperator::residualNorm()
         setup_->setScannerGeometry(geometries_[phys]);
         forward_->SetOutput(&virtual_[0], &traversal_[0]);
         for (uint32_t s = 0; s < subvolumes; ++s) forward_->Execute(s);
-        dev::launch_sum_squared_difference(realProjection(phys), virtual_[0].data, pixels_, partials, total, stream_);
+        dev::launch_sum_squared_difference(realProjection(phys), projection_type_, virtual_[0].data, pixels_, partials,
+                                           total, stream_);
     }
     double host = 0.0;
     ETN_HIP(hipMemcpyAsync(&host, total, sizeof(double), hipMemcpyDeviceToHost, stream_));

@@ -64,7 +64,12 @@ public:
     uint32_t getBlockSize(uint32_t block) const;
     uint32_t getPhysicalProjectionIndex(uint32_t block, uint32_t p) const;
     const ParallelScannerGeometry& geometry(uint32_t physical) const { return geometries_.at(physical); }
-    const float* realProjection(uint32_t physical) const { return projections_ + (size_t)physical * pixels_; }
+    // measured projection `physical` in its storage type (projectionType())
+    const void* realProjection(uint32_t physical) const
+    {
+        return static_cast<const char*>(projections_) + (size_t)physical * pixels_ * voxel_bytes(projection_type_);
+    }
+    etn_voxel_type projectionType() const { return projection_type_; }
     // Physical indices of the projections the residual norm runs over (all but the skip list).
     const std::vector<uint32_t>& measuredProjections() const { return measured_; }
 
@@ -85,7 +90,8 @@ private:
     uint32_t block_count_ = 0;
     std::unique_ptr<ProjectionSet> projection_set_;
     std::vector<ParallelScannerGeometry> geometries_;     // per physical projection
-    const float* projections_;
+    const void* projections_;
+    etn_voxel_type projection_type_ = ETN_VOXEL_FLOAT32;   // f16/bf16 with ETN_RECON_FLAG_16BIT_PROJECTIONS
     size_t pixels_;
 
     std::unique_ptr<GpuVolume> volume_;

@@ -352,6 +352,7 @@ void build_passes(MemberPlan& mb)
             g.vertical_pitch().store(f.v);
             g.ray_direction().store(f.ray);
             f.measured = op.realProjection(phys);
+            f.measured_type = op.projectionType();
             f.residual = image(parity, p, 0);
             f.length = image(parity, p, 1);
             f.loc_c = loc_c;
@@ -633,6 +634,7 @@ double FusedEngineImpl::residualNorm(size_t member, double* scratch, hipStream_t
             g.vertical_pitch().store(f.v);
             g.ray_direction().store(f.ray);
             f.measured = op.realProjection(measured[k]);
+            f.measured_type = op.projectionType();
             f.residual = slot((int)(k - c0), 0);
             f.length = slot((int)(k - c0), 1);
             f.raw = 1;
@@ -672,7 +674,8 @@ double FusedEngineImpl::residualNorm(size_t member, double* scratch, hipStream_t
         // squared differences in projection order, as the unfused norm adds them
         const size_t c0 = c * (size_t)max_fp;
         for (size_t k = c0; k < std::min(measured.size(), c0 + (size_t)max_fp); ++k)
-            dev::launch_sum_squared_difference(op.realProjection(measured[k]), slot((int)(k - c0), 0), m.pixels,
+            dev::launch_sum_squared_difference(op.realProjection(measured[k]), op.projectionType(),
+                                               slot((int)(k - c0), 0), m.pixels,
                                                partials, total, stream);
     }
     double host = 0.0;

@@ -79,7 +79,8 @@ __host__ __device__ inline int shape_band_floats(int threads, bool mask, int vby
 
 struct FusedFp {
     float src[3], h[3], v[3], ray[3];
-    const float* measured;       // nv x nu
+    const void* measured;        // nv x nu, in measured_type (f32, or f16/bf16 stored projections)
+    int measured_type;
     float* residual;             // nv x nu, written: measured - line integral (raw: the line integral)
     float* length;               // nv x nu, written
     float loc_c;
@@ -833,6 +834,14 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         __syncthreads();
     }
 
+    // the measured projection's value in its storage type (a uniform branch)
+    auto measured_at = [](const ETN_GLOBAL FusedFp& f, size_t i) -> float {
+        if (f.measured_type == ETN_VOXEL_FLOAT16) return F16::load(global_ptr(static_cast<const __half*>(f.measured)), i);
+        if (f.measured_type == ETN_VOXEL_BFLOAT16)
+            return BF16::load(global_ptr(static_cast<const uint16_t*>(f.measured)), i);
+        return global_ptr(static_cast<const float*>(f.measured))[i];
+    };
+
     // ---- residual (+ long-object compensation) per pixel ----
     // oversampled: every traced sub-ray's sums go through LDS (free after the
     // last band); the pixel's first slot adds sub-ray (si, sj)'s sums, those of
@@ -873,7 +882,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                 l = f.loc_c;
             }
             const size_t i = (size_t)w * nu + u;
-            global_ptr(f.residual)[i] = f.raw ? pv : global_ptr(f.measured)[i] - pv;
+            global_ptr(f.residual)[i] = f.raw ? pv : measured_at(f, i) - pv;
             global_ptr(f.length)[i] = l;
         }
     }
@@ -892,7 +901,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                         l = f.loc_c;
                     }
                     const size_t i = (size_t)w * nu + u;
-                    global_ptr(f.residual)[i] = f.raw ? pv : global_ptr(f.measured)[i] - pv;
+                    global_ptr(f.residual)[i] = f.raw ? pv : measured_at(f, i) - pv;
                     global_ptr(f.length)[i] = l;
                 }
             }

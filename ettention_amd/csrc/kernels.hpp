@@ -42,12 +42,18 @@ void launch_forward_project(const SlabParams& s, const RayParams& r, etn_voxel_t
 void launch_back_project(const SlabParams& s, const BpParams& b, etn_voxel_type vt, void* volume,
                          const uint8_t* mask, const float* residual, const float* ray_length, hipStream_t stream);
 void launch_compare(uint32_t n, const float* real, const float* virt, float* residual, hipStream_t stream);
+// real in its storage type (measured projections stored as f16 or bf16)
+void launch_compare(uint32_t n, const void* real, etn_voxel_type real_type, const float* virt, float* residual,
+                    hipStream_t stream);
 void launch_long_object_compensate(uint32_t n, float* projection, float* length, float c, hipStream_t stream);
 void launch_ray_length(const float bmin[3], const float bmax[3], const RayParams& r, float* out, hipStream_t stream);
 // partials must hold ssd_partial_count() doubles
 constexpr uint32_t kSsdBlocks = 512;
 void launch_sum_squared_difference(const float* a, const float* b, size_t count, double* partials, double* out,
                                    hipStream_t stream);
+// a in its storage type
+void launch_sum_squared_difference(const void* a, etn_voxel_type a_type, const float* b, size_t count,
+                                   double* partials, double* out, hipStream_t stream);
 // Device address of `device`'s static partial-sum array (kSsdBlocks doubles),
 // looked up once per device with that device current.
 double* ssd_static_scratch(int device);

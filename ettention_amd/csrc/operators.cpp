@@ -268,7 +268,7 @@ void CompareKernel::run()
     if (!real_ || !virt_ || !residual_) fail("CompareKernel needs input and output images");
     if (real_->pixels() != virt_->pixels() || real_->pixels() != residual_->pixels())
         fail("CompareKernel images differ in size");
-    dev::launch_compare((uint32_t)residual_->pixels(), real_->data, virt_->data, residual_->data, stream_);
+    dev::launch_compare((uint32_t)residual_->pixels(), real_->data, real_type_, virt_->data, residual_->data, stream_);
 }
 
 void VirtualProjectionKernel::run()

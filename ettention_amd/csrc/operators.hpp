@@ -209,11 +209,19 @@ private:
 class CompareKernel {
 public:
     explicit CompareKernel(hipStream_t stream) : stream_(stream) {}
-    void setInput(const DeviceImage* real, const DeviceImage* virt) { real_ = real; virt_ = virt; }
+    // real_type: storage of the measured image (f16/bf16 projections; its data
+    // pointer then addresses 16-bit values)
+    void setInput(const DeviceImage* real, const DeviceImage* virt, etn_voxel_type real_type = ETN_VOXEL_FLOAT32)
+    {
+        real_ = real;
+        virt_ = virt;
+        real_type_ = real_type;
+    }
     void setOutput(DeviceImage* residual) { residual_ = residual; }
     void run();
 
 private:
+    etn_voxel_type real_type_ = ETN_VOXEL_FLOAT32;
     const DeviceImage* real_ = nullptr;
     const DeviceImage* virt_ = nullptr;
     DeviceImage* residual_ = nullptr;

@@ -80,10 +80,12 @@ __global__ void __launch_bounds__(256) back_project_kernel(SlabParams s, BpParam
     S::store(vol, i, value);
 }
 
-__global__ void compare_kernel(uint32_t n, const float* real, const float* virt, float* res)
+// the measured projection in its storage type (f32, or f16/bf16 stored projections)
+template <typename S>
+__global__ void compare_kernel(uint32_t n, const typename S::T* real, const float* virt, float* res)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) res[i] = real[i] - virt[i];
+    if (i < n) res[i] = S::load(real, i) - virt[i];
 }
 
 __global__ void loc_kernel(uint32_t n, float* proj, float* len, float c)
@@ -123,13 +125,14 @@ __global__ void ray_length_kernel(RayParams g, float bx0, float by0, float bz0, 
 // FP64 sum of squared differences: per-block partials, then one block sums
 // the partials in a fixed order (deterministic).
 constexpr int kSsdThreads = 256;
-__global__ void __launch_bounds__(kSsdThreads) ssd_partial_kernel(const float* a, const float* b, size_t n,
+template <typename S>
+__global__ void __launch_bounds__(kSsdThreads) ssd_partial_kernel(const typename S::T* a, const float* b, size_t n,
                                                                   double* partials)
 {
     __shared__ double red[kSsdThreads];
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double d = (double)a[i] - (b ? (double)b[i] : 0.0);
+        const double d = (double)S::load(a, i) - (b ? (double)b[i] : 0.0);
         acc += d * d;
     }
     red[threadIdx.x] = acc;
@@ -237,10 +240,22 @@ void launch_back_project(const SlabParams& s, const BpParams& b, etn_voxel_type 
     check_launch("back projection launch");
 }
 
+void launch_compare(uint32_t n, const void* real, etn_voxel_type real_type, const float* virt, float* residual,
+                    hipStream_t stream)
+{
+    const uint32_t grid = (n + 255) / 256;
+    if (real_type == ETN_VOXEL_FLOAT16)
+        compare_kernel<F16><<<grid, 256, 0, stream>>>(n, (const __half*)real, virt, residual);
+    else if (real_type == ETN_VOXEL_BFLOAT16)
+        compare_kernel<BF16><<<grid, 256, 0, stream>>>(n, (const uint16_t*)real, virt, residual);
+    else
+        compare_kernel<F32><<<grid, 256, 0, stream>>>(n, (const float*)real, virt, residual);
+    check_launch("compare launch");
+}
+
 void launch_compare(uint32_t n, const float* real, const float* virt, float* residual, hipStream_t stream)
 {
-    compare_kernel<<<(n + 255) / 256, 256, 0, stream>>>(n, real, virt, residual);
-    check_launch("compare launch");
+    launch_compare(n, real, ETN_VOXEL_FLOAT32, virt, residual, stream);
 }
 
 void launch_long_object_compensate(uint32_t n, float* projection, float* length, float c, hipStream_t stream)
@@ -257,10 +272,15 @@ void launch_ray_length(const float bmin[3], const float bmax[3], const RayParams
     check_launch("ray length launch");
 }
 
-void launch_sum_squared_difference(const float* a, const float* b, size_t count, double* partials, double* out,
-                                   hipStream_t stream)
+void launch_sum_squared_difference(const void* a, etn_voxel_type a_type, const float* b, size_t count,
+                                   double* partials, double* out, hipStream_t stream)
 {
-    ssd_partial_kernel<<<kSsdBlocks, kSsdThreads, 0, stream>>>(a, b, count, partials);
+    if (a_type == ETN_VOXEL_FLOAT16)
+        ssd_partial_kernel<F16><<<kSsdBlocks, kSsdThreads, 0, stream>>>((const __half*)a, b, count, partials);
+    else if (a_type == ETN_VOXEL_BFLOAT16)
+        ssd_partial_kernel<BF16><<<kSsdBlocks, kSsdThreads, 0, stream>>>((const uint16_t*)a, b, count, partials);
+    else
+        ssd_partial_kernel<F32><<<kSsdBlocks, kSsdThreads, 0, stream>>>((const float*)a, b, count, partials);
     ssd_final_kernel<<<1, kSsdThreads, 0, stream>>>(partials, kSsdBlocks, out);
     check_launch("residual norm launch");
 }
@@ -291,6 +311,12 @@ double* ssd_static_scratch(int device)
         address[device] = static_cast<double*>(p);
     }
     return address[device];
+}
+
+void launch_sum_squared_difference(const float* a, const float* b, size_t count, double* partials, double* out,
+                                   hipStream_t stream)
+{
+    launch_sum_squared_difference(a, ETN_VOXEL_FLOAT32, b, count, partials, out, stream);
 }
 
 void launch_convert(const void* src, etn_voxel_type st, void* dst, etn_voxel_type dt, size_t count, hipStream_t stream)

@@ -60,10 +60,11 @@ dev::RampFilterParams WeightedBackProjection::filterParams(uint32_t nu, uint32_t
 }
 
 WeightedBackProjection::WeightedBackProjection(const etn_reconstruction_desc& d, hipStream_t stream)
-    : count_(d.projection_count), lambda_(d.lambda), projections_(d.projections)
+    : count_(d.projection_count), lambda_(d.lambda), projections_(static_cast<const float*>(d.projections))
 {
     if (d.projection_count == 0 || (!d.tilt_angles && !d.geometries)) fail("No projections given!");
     if (!d.projections) fail("projection stack device pointer is null");
+    if (d.flags & ETN_RECON_FLAG_16BIT_PROJECTIONS) unsupported("WBP filters f32 projections");
     const uint32_t nu = d.base_geometry.resolution[0], nv = d.base_geometry.resolution[1];
     if (nu == 0 || nv == 0) fail("detector resolution is zero");
     pixels_ = (size_t)nu * nv;

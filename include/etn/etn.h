@@ -193,7 +193,12 @@ enum {
                                                    of to nearest even.  Extension (the reference rounds to
                                                    nearest); needs the fast fused kernels: f16 or bf16 storage,
                                                    no mask, no oversampling, nx a multiple of 4, nx + nz <= 2049;
-                                                   creation fails otherwise */
+                                                   creation fails otherwise */,
+    ETN_RECON_FLAG_16BIT_PROJECTIONS = 1u << 5   /* the measured projections are stored in the volume's 16-bit
+                                                   type (voxel_type f16 or bf16): `projections` addresses
+                                                   projection_count * nu * nv such values.  Arithmetic stays
+                                                   FP32.  Extension (the reference keeps projections in f32);
+                                                   not for "wbp" */
 };
 
 /* Parameters of BlockIterativeReconstructionOperator, named after the
@@ -218,7 +223,8 @@ typedef struct etn_reconstruction_desc {
     uint32_t projection_count;
     const float* tilt_angles;           /* host, degrees, projection_count entries */
     float x_axis_tilt;                  /* input.xAxisTilt */
-    const float* projections;           /* device, projection_count * nu * nv, stack order */
+    const void* projections;            /* device, projection_count * nu * nv floats in stack order
+                                           (16-bit values with ETN_RECON_FLAG_16BIT_PROJECTIONS) */
     const uint8_t* mask;                /* device prior-knowledge mask or NULL */
     uint32_t flags;                     /* ETN_RECON_FLAG_* */
     /* host, projection_count entries, or NULL: the geometry of each projection

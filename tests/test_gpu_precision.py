@@ -69,3 +69,47 @@ def test_stochastic_rounding_needs_the_fast_16bit_kernels():
             A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
                                                    projections=projs, voxel_type=vt,
                                                    flags=E.ETN_RECON_FLAG_STOCHASTIC_ROUNDING | extra)
+
+
+def _case(n=64, seed=21):
+    vg = E.VolumeGeometry.make((n, n, n))
+    base = E.ScannerGeometry.mrc_base(n, n)
+    tilts = S.tilt_series(60.0, 4.0)
+    ph = S.ellipsoid_phantom((n, n, n), seed=seed, device=torch.device("cuda"))
+    projs = torch.zeros((len(tilts), n, n), dtype=torch.float32, device="cuda")
+    scratch = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    for i, t in enumerate(tilts):
+        scratch.zero_()
+        A.forward_project(vg, ph, A.rotate(base, float(t)), projs[i], scratch)
+    return vg, base, tilts, S.add_gaussian_noise(projs, 0.02, seed=seed)
+
+
+@pytest.mark.parametrize("voxel_type", ["bfloat16", "half"])
+@pytest.mark.parametrize("flags", [0, E.ETN_RECON_FLAG_EXACT_ARITHMETIC, E.ETN_RECON_FLAG_GENERIC_KERNELS])
+def test_16bit_projections_equal_f32_projections_of_the_same_values(voxel_type, flags):
+    """Measured projections stored in the volume's 16-bit type give the same
+    volume and residual norm, bit for bit, as f32 projections holding the same
+    (rounded) values: only the storage differs."""
+    vg, base, tilts, projs = _case()
+    p16 = projs.to(torch.bfloat16 if voxel_type == "bfloat16" else torch.float16).contiguous()
+    p32 = p16.float().contiguous()
+    out = []
+    for p in (p16, p32):
+        r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts, projections=p,
+                                                   algorithm="sart", lam=0.3, projection_iteration="random",
+                                                   voxel_type=voxel_type, flags=flags)
+        assert r.uses_fused_path() == (flags != E.ETN_RECON_FLAG_GENERIC_KERNELS)
+        r.run_iterations(2)
+        out.append((r.get_reconstructed_volume(dtype=torch.float32), r.residual_norm()))
+    assert torch.equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_16bit_projections_need_16bit_storage():
+    vg, base, tilts, projs = _case(32)
+    with pytest.raises(ValueError):
+        A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                               projections=projs.to(torch.bfloat16), voxel_type="float")
+    with pytest.raises(ValueError):   # the projections take the volume's 16-bit type
+        A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts,
+                                               projections=projs.to(torch.float16), voxel_type="bfloat16")
