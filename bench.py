@@ -1,8 +1,8 @@
 """SART throughput on one MI355X: GUPS on the 512^3 x 121-tilt workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxel-type float|half|bfloat16]
-                    [--workload metric|c2|c3|c4|c5] [--generic] [--mask] [--fp-samples N]
-                    [--bp-samples M] [--no-cpu-baseline]
+                    [--workload metric|c2|c3|c4|c5|wbp] [--generic] [--lds-back-projection] [--mask]
+                    [--fp-samples N] [--bp-samples M] [--no-cpu-baseline]
 
 A step is one full iteration of the block-iterative loop over the synthetic
 tilt series (for SART: every projection's forward projection, residual and
@@ -61,6 +61,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def run_flags(args):
+    import ettention_amd as E
+    return ((E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0) |
+            (E.ETN_RECON_FLAG_LDS_BACK_PROJECTION if args.lds_back_projection else 0))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +75,10 @@ def main():
     ap.add_argument("--voxel-type", default="float", choices=["float", "half", "bfloat16"])
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--generic", action="store_true", help="force the generic (unfused) kernels")
+    ap.add_argument("--lds-back-projection", action="store_true",
+                    help="fused kernels: back projections on the band in LDS instead of in registers")
+    ap.add_argument("--workgroup-threads", type=int, default=0,
+                    help="metric/c2: fused kernels' workgroup shape (256 or 512) instead of the autotuner's")
     ap.add_argument("--mask", action="store_true", help="metric/c2: reconstruct with a prior-knowledge mask")
     ap.add_argument("--fp-samples", type=int, default=1, help="metric/c2: forward sub-rays per pixel axis")
     ap.add_argument("--bp-samples", type=int, default=1, help="metric/c2: back projection samples per voxel")
@@ -132,7 +142,7 @@ def main():
     del phantom, scratch
 
     mask = support_mask(nx, ny, nz, dev) if args.mask else None
-    flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
+    flags = run_flags(args)
     # the reconstruction runs on a stream of its own, on which the timed
     # region's HIP events are recorded
     stream = torch.cuda.Stream()
@@ -143,6 +153,8 @@ def main():
                                                fp_samples=args.fp_samples, bp_samples=args.bp_samples,
                                                stream=stream.cuda_stream)
     fused = r.uses_fused_path()
+    if args.workgroup_threads:
+        r.set_workgroup_threads(args.workgroup_threads)
     for _ in range(args.warmup):
         r.run_single_iteration()
     r.synchronize()
@@ -444,7 +456,7 @@ def run_wbp(args):
     projs = S.add_gaussian_noise(projs, 0.02, seed=7)
     del phantom, scratch
     torch.cuda.synchronize()
-    flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
+    flags = run_flags(args)
     r = A.WeightedBackProjection(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs, lam=wl["lam"],
                                  flags=flags)
     zero = torch.zeros((nz, ny, nx), dtype=torch.float32, device=dev)
@@ -546,7 +558,7 @@ def run_batch(args):
         del ph, scratch
     torch.cuda.synchronize()
     log("synthetic batch of %d members ready in %.1f s" % (len(members), time.time() - t0))
-    flags = E.ETN_RECON_FLAG_GENERIC_KERNELS if args.generic else 0
+    flags = run_flags(args)
     b = A.BatchReconstruction(members, flags=flags)
     fused = all(b.uses_fused_path(i) for i in range(len(b)))
     for _ in range(args.warmup):

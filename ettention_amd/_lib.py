@@ -29,6 +29,7 @@ ETN_RECON_FLAG_NO_AUTOTUNE = 4
 ETN_RECON_FLAG_EXACT_ARITHMETIC = 8
 ETN_RECON_FLAG_STOCHASTIC_ROUNDING = 16
 ETN_RECON_FLAG_16BIT_PROJECTIONS = 32
+ETN_RECON_FLAG_LDS_BACK_PROJECTION = 64
 
 
 class EtnError(RuntimeError):

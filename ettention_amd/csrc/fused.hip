@@ -61,6 +61,7 @@ struct MemberPlan {
     int vbytes = 4;                    // bytes of a stored voxel (the 512-thread shape's LDS array depends on it)
     int rpt_cap = dev::kMaxRaysPerThread;  // rays per thread a pass may give its forward projections
     bool sr = false;                   // stochastic rounding of the back projections' updates to 16-bit storage
+    bool lds_bp = false;               // back projections on the band in LDS, never in registers
     int fp_n = 1, fp_shift = 0;        // forward sub-rays per pixel axis; log2 of ray slots (columns si) per pixel
     int bp_samples = 1, rowsets = 1;   // back projection samples per voxel; pattern rows
     bool oversampled() const { return fp_n != 1 || bp_samples != 1; }
@@ -419,6 +420,8 @@ private:
         int rpt;
         size_t lds;
         bool steady;
+        int res;                // register back projection (dev::FusedLaunch::res)
+        size_t lds_res;         // LDS bytes with every back projection's records staged
     };
     struct Schedule {
         std::vector<Launch> launches;
@@ -433,7 +436,6 @@ private:
     void useRptCap(int cap);
     size_t passCount() const;
     void tuneTiles(hipStream_t stream);
-    float timeLaunches(const Schedule& s, hipStream_t stream) const;
     float timeSchedule(const Schedule& s, hipStream_t stream) const;
 
     std::vector<MemberPlan> members_;
@@ -442,6 +444,7 @@ private:
     bool mask_;                         // every member has a mask (the launches use the mask kernels)
     bool over_;                         // every member oversamples (the launches use the oversampled kernels)
     bool sr_;                           // stochastic rounding to 16-bit storage
+    bool reg_bp_;                       // register back projection where it applies
     int threads_ = dev::kMaxThreads;    // workgroup shape of every launch
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
@@ -451,7 +454,7 @@ private:
 FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type vt, bool vec, bool cap, bool exact,
                                  bool mask, bool over, hipStream_t stream)
     : members_(std::move(members)), vt_(vt), vec_(vec), cap_(cap), exact_(exact), mask_(mask), over_(over),
-      sr_(!members_.empty() && members_[0].sr)
+      sr_(!members_.empty() && members_[0].sr), reg_bp_(!members_.empty() && !members_[0].lds_bp)
 {
     diag_.allocate(sizeof(int));
     ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
@@ -560,7 +563,9 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
     std::vector<const dev::FusedPass*> tables(length * members_.size(), nullptr);
     out.launches.clear();
     for (size_t k = 0; k < length; ++k) {
-        Launch l{work.size(), 0, 1, 0, true};
+        Launch l{work.size(), 0, 1, 0, true, 0, 0};
+        bool fit_shape = true, fit_res = true;
+        int max_bp = 0;
         for (size_t i : order) {
             const MemberPlan& m = members_[i];
             if (k < (size_t)m.offset || k >= (size_t)m.offset + m.passes.size()) continue;
@@ -574,7 +579,22 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
                                                                                    m.row_floats, mask_, threads_,
                                                                                    m.rowsets, m.vbytes));
             l.steady = l.steady && p.nfp > 0 && p.nbp > 0;
+            max_bp = std::max(max_bp, p.nbp);
+            const size_t res = sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, m.tile.band, m.nx,
+                                                                             m.row_floats, mask_, threads_,
+                                                                             m.rowsets * std::max(p.nbp, 1), m.vbytes);
+            l.lds_res = std::max(l.lds_res, res);
+            // the register path needs a thread's groups at one x in every band row
+            const bool same_x = threads_ % (m.tile.planes * m.nx / 4) == 0;
+            fit_shape = fit_shape && same_x && res <= sizeof(float) * (size_t)dev::shape_lds_floats(threads_, m.vbytes);
+            fit_res = fit_res && same_x && res <= sizeof(float) * (size_t)dev::kResLds;
         }
+        // passes with one back projection keep the LDS path: in registers it
+        // gained nothing with f32 storage (638 against 636 GUPS at 512^3) and
+        // lost 13% with bf16 (the band's values and the next band's loads
+        // exceed the VGPR budget of four waves per SIMD)
+        if (reg_bp_ && !exact_ && !over_ && vec_ && !cap_ && !sr_ && max_bp > 1)
+            l.res = fit_shape ? 1 : threads_ == 512 && fit_res ? 2 : 0;
         l.groups = (int)(work.size() - l.work);
         out.launches.push_back(l);
     }
@@ -597,7 +617,7 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         const Launch& l = s.launches[k];
         if (l.groups == 0) continue;
         const dev::FusedLaunch fl{threads_, l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
-                                  tables + k * members_.size(), l.lds, sr_};
+                                  tables + k * members_.size(), l.res ? l.lds_res : l.lds, sr_, l.res};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
         dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
@@ -765,28 +785,6 @@ void FusedEngineImpl::setThreads(int threads, hipStream_t stream)
     upload(stream);
 }
 
-// Event time of launches 1 .. kTimed of a schedule (launch 0 runs untimed); -1
-// when the schedule has fewer than two launches.
-float FusedEngineImpl::timeLaunches(const Schedule& s, hipStream_t stream) const
-{
-    constexpr size_t kTimed = 4;
-    const size_t n = s.launches.size();
-    if (n < 2) return -1.0f;
-    hipEvent_t a, b;
-    ETN_HIP(hipEventCreate(&a));
-    ETN_HIP(hipEventCreate(&b));
-    enqueue(s, 0, 1, stream, nullptr);
-    ETN_HIP(hipEventRecord(a, stream));
-    enqueue(s, 1, std::min(n, 1 + kTimed), stream, nullptr);
-    ETN_HIP(hipEventRecord(b, stream));
-    ETN_HIP(hipEventSynchronize(b));
-    float ms = 0.0f;
-    ETN_HIP(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return ms;
-}
-
 // Every launch of a schedule, after one untimed launch.
 float FusedEngineImpl::timeSchedule(const Schedule& s, hipStream_t stream) const
 {
@@ -808,7 +806,10 @@ float FusedEngineImpl::timeSchedule(const Schedule& s, hipStream_t stream) const
 }
 
 // Members of one size share a tile; each size is timed on its own, under the
-// current workgroup shape.
+// current workgroup shape, over a whole iteration: the planes of a tile decide
+// how many forward projections a pass takes (the rays per workgroup are
+// capped), so the tiles cut an iteration into different passes, and a few
+// launches of each would not compare equal work.
 void FusedEngineImpl::tuneTiles(hipStream_t stream)
 {
     // (the staged row length and row sets and the ray slots per pixel decide
@@ -830,7 +831,7 @@ void FusedEngineImpl::tuneTiles(hipStream_t stream)
             for (size_t i : ids) members_[i].offset = 0;
             Schedule s;
             build_schedule(ids, s, stream);
-            const float ms = timeLaunches(s, stream);
+            const float ms = timeSchedule(s, stream);
             if (ms >= 0.0f && (best_ms < 0.0f || ms < best_ms)) {
                 best_ms = ms;
                 best = t;
@@ -1007,8 +1008,18 @@ void FusedBackProjectorImpl::backProject(const std::vector<uint32_t>& indices,
     std::vector<hipEvent_t> ev(launch_ms ? 2 * passes.size() : 0);
     for (auto& e : ev) ETN_HIP(hipEventCreate(&e));
     for (size_t k = 0; k < passes.size(); ++k) {
+        // register back projection: every record set of the pass staged at once
+        const size_t lds_res = sizeof(float) * (size_t)dev::fused_lds_floats(
+                                   m_.tile.planes, m_.tile.band, m_.nx, m_.row_floats, false, m_.threads,
+                                   m_.rowsets * std::max(passes[k].nbp, 1), m_.vbytes);
+        int res = 0;
+        if (!m_.lds_bp && !m_.oversampled() && passes[k].nbp > 1 && m_.threads % (m_.tile.planes * m_.nx / 4) == 0)
+            res = lds_res <= sizeof(float) * (size_t)dev::shape_lds_floats(m_.threads, m_.vbytes) ? 1
+                  : m_.threads == 512 && lds_res <= sizeof(float) * (size_t)dev::kResLds           ? 2
+                                                                                                     : 0;
         const dev::FusedLaunch fl{m_.threads, 1, (int)work.size(), dvol.as<dev::FusedVolume>(),
-                                  dwork.as<dev::FusedWork>(), dtables.as<const dev::FusedPass*>() + k, lds};
+                                  dwork.as<dev::FusedWork>(), dtables.as<const dev::FusedPass*>() + k,
+                                  res ? lds_res : lds, false, res};
         if (launch_ms) ETN_HIP(hipEventRecord(ev[2 * k], stream));
         dev::launch_fused(volume_->voxel_type(), true, false, false, false, m_.oversampled(), fl, stream);
         if (launch_ms) ETN_HIP(hipEventRecord(ev[2 * k + 1], stream));
@@ -1027,7 +1038,7 @@ void FusedBackProjectorImpl::backProject(const std::vector<uint32_t>& indices,
 std::unique_ptr<FusedBackProjector> FusedBackProjector::create(GpuVolume* volume,
                                                                const std::vector<ParallelScannerGeometry>& geometries,
                                                                uint32_t nu, uint32_t nv, uint32_t bp_samples,
-                                                               hipStream_t)
+                                                               bool lds_bp, hipStream_t)
 {
     const VolumeProperties vp = whole_volume(volume->properties());
     MemberPlan m;
@@ -1058,6 +1069,7 @@ std::unique_ptr<FusedBackProjector> FusedBackProjector::create(GpuVolume* volume
     m.vbytes = (int)voxel_bytes(volume->voxel_type());
     m.bp_samples = bp_s;
     m.rowsets = pd;
+    m.lds_bp = lds_bp;
     if (dev::fused_lds_floats(1, 1, m.nx, m.row_floats, false, m.threads, m.rowsets, m.vbytes) >
         dev::shape_lds_floats(m.threads, m.vbytes) || max_band(m, 1) < 1)
         return nullptr;
@@ -1076,7 +1088,9 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
         const etn_reconstruction_desc& d = *members[i].desc;
         const bool d_over = (d.fp_samples ? d.fp_samples : 1) != 1 || (d.bp_samples ? d.bp_samples : 1) != 1;
         const bool d_sr = (d.flags & ETN_RECON_FLAG_STOCHASTIC_ROUNDING) != 0;
+        const bool d_lds = (d.flags & ETN_RECON_FLAG_LDS_BACK_PROJECTION) != 0;
         if (!plans.empty() && (d.voxel_type != vt || (d.mask != nullptr) != mask || d_over != over || d_sr != sr ||
+                               d_lds != plans[0].lds_bp ||
                                (members[i].op->getReconstructedVolume()->properties().nx() % 4 == 0) != vec))
             continue;
         // stochastic rounding exists on the fast 16-bit kernels without mask or oversampling
@@ -1087,6 +1101,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
         if (!plan_member(*members[i].op, d, exact, m)) continue;
         if (d_sr && m.nx + m.nz > 2049) continue;
         m.sr = d_sr;
+        m.lds_bp = d_lds;
         if (plans.empty()) {
             vt = d.voxel_type;
             vec = m.nx % 4 == 0;

@@ -28,9 +28,12 @@ class FusedBackProjector {
 public:
     // Null when the geometries have no fused form (x-axis tilt, nx not a
     // multiple of 4, a sample count the fused kernels do not take).
+    // lds_bp: apply the projections to the band in LDS one after another,
+    // never in registers as the band is loaded (same volumes).
     static std::unique_ptr<FusedBackProjector> create(GpuVolume* volume,
                                                       const std::vector<ParallelScannerGeometry>& geometries,
-                                                      uint32_t nu, uint32_t nv, uint32_t bp_samples, hipStream_t stream);
+                                                      uint32_t nu, uint32_t nv, uint32_t bp_samples, bool lds_bp,
+                                                      hipStream_t stream);
     virtual ~FusedBackProjector() = default;
     // Back-projects projections indices[k] (into the geometry list) with
     // residual[k] and ray length[k] (device nv x nu images) and weight

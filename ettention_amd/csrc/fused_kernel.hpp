@@ -65,6 +65,22 @@ struct FusedShape<256, kBytes> {
     static constexpr int kLds = 10240;
     static constexpr int kBand = 6144;
 };
+// The register back projection with every record set of a pass resident in
+// LDS (kernels with kRes == 2, 512 threads only): a 160 KiB array, so one
+// workgroup per CU (the 512-thread kernels with 4 or 8 rays per thread hold
+// only one anyway, by their registers), with the band of the 512-thread shape.
+constexpr int kResLds = 40960;
+template <int kThreads, int kBytes, bool kResident>
+struct FusedShapeOf {
+    using type = FusedShape<kThreads, kBytes>;
+};
+template <int kBytes>
+struct FusedShapeOf<512, kBytes, true> {
+    struct type {
+        static constexpr int kLds = kResLds;
+        static constexpr int kBand = FusedShape<512, kBytes>::kBand;
+    };
+};
 constexpr int kMaxThreads = 512;
 __host__ __device__ inline int shape_lds_floats(int threads, int vbytes)
 {
@@ -157,6 +173,11 @@ struct FusedLaunch {
     const FusedPass* const* passes;
     size_t lds;
     bool sr;                     // stochastic rounding to 16-bit storage (fast kernels, no mask or oversampling)
+    // back projections applied in registers as the band is loaded (fast
+    // kernels, nx a multiple of 4, no oversampling, stochastic rounding or
+    // step limit), the record sets of all of a pass's back projections staged
+    // once per launch: 0 off, 1 in the shape's LDS array, 2 in the 160 KiB one
+    int res;
 };
 
 namespace {
@@ -350,6 +371,33 @@ struct GroupIo<S, false> {
 __device__ __forceinline__ int padded_index(int i0, int n) { return min(max(i0, -kRowPad), n) + kRowPad; }
 __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a + d * t; }
 
+// One single-sample back projection update of a voxel with fast arithmetic
+// (ParallelBeamsBackProjection.cl:26-52): xt + zt = m0 px + m4 py + m8 pz, u
+// in the reference's order (u is a few hundred pixels, so a reordered sum
+// would move the weights by ~1e-4); fract without the clamp below 1 (at tx == 1
+// the interpolation gives the next column's value, its limit).  pr: the
+// plane's staged records, padded (on the host) beyond every column the
+// member's projections reach, so the index needs no clamp.  lc: lambda times
+// the in-voxel chord; mv: the mask weight; vid: the voxel's index in the
+// volume (stochastic rounding's random bits).
+template <typename S, bool kMask, bool kSR>
+__device__ __forceinline__ float bp_update_fast(const float4* pr, int pad_l, float xt, float zt, float m12, float lc,
+                                                float value, float mv, uint32_t vid, uint32_t salt)
+{
+    float u = xt + zt;
+    u = u + m12 * 1.0f;
+    u = u - 0.5f;
+    const float bu = floorf(u);
+    const float tx = u - bu;
+    const float4 c = pr[(int)bu + pad_l];
+    const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
+    const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
+    const float next = kMask ? fmaf(mv, contrib, value) : value + contrib;
+    value = (L > kEpsilon) ? next : value;
+    if constexpr (kSR) return S::round_stochastic(value, sr_hash(vid, salt, value));
+    return S::round(value);
+}
+
 // One workgroup per (member, plane group) work item.  `passes[m]` is member m's
 // pass of this launch; members without one get no work items.
 //
@@ -373,13 +421,24 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // and adds its sums fp_n times, in the generic kernel's (si, sj) order.
 // With at most two rays per thread the kernel fits 128 VGPRs: four waves per
 // SIMD, two 512-thread workgroups per CU (as LDS also allows).
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT, bool kSR>
-__global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
+// kRes != 0 (fast arithmetic, nx a multiple of 4, no oversampling or
+// stochastic rounding) applies the back projections to each group of 4 voxels
+// in registers as the band arrives from HBM, all of a pass's back projections
+// in block order, before the group goes to LDS: no LDS read and write of the
+// band per back projection and no barrier between them.  The record sets of
+// every back projection of the pass are staged once per launch (kRes == 2: in
+// the 160 KiB array).  Every voxel sees the same chain of updates and
+// roundings as on the LDS path, so the volumes are bit-identical.
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT, bool kSR,
+          int kRes>
+__global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
 {
+    static_assert(kRes == 0 || (kVec && kFast && !kOver && !kSR), "register back projection: fast vector kernels");
+    static_assert(kRes != 2 || kThreads == 512, "resident record sets: 512-thread workgroups");
     using IO = GroupIo<S, kVec>;
-    using Shape = FusedShape<kThreads, (int)sizeof(typename S::T)>;
+    using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes == 2>::type;
     constexpr int G = IO::G;
     constexpr int kMaskOff = Shape::kBand / 2;          // LDS floats from a voxel to its mask weight
     constexpr int kBandCap = kMask ? kMaskOff : Shape::kBand;
@@ -402,7 +461,6 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     float* mband = lds + kMaskOff;                      // kMask: the band's mask weights
     float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskOff : 0) + ((fv.band * width + 3) & ~3));
     constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
-    float2* pterm = reinterpret_cast<float2*>(recs + np * rowsets * kRecStride * row_len);
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
     const ETN_GLOBAL uint8_t* mask = kMask ? global_ptr(fv.mask) : nullptr;
     // MaskVolume weight of a byte, as the generic kernels compute it
@@ -412,6 +470,11 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
     const int gpr = width / G;                          // groups per band row
 
     const int nfp = pass->nfp, nbp = pass->nbp;
+    // record sets staged at a time (set q: back projection q; the LDS path
+    // restages set 0 per projection when a pass has several)
+    const int nsets = kRes ? max(nbp, 1) : 1;
+    const int set_len = np * rowsets * kRecStride * row_len;   // float4 per record set
+    float2* pterm = reinterpret_cast<float2*>(recs + nsets * set_len);
     const int R = fv.rows_per_plane;
     const int fp_n = kOver ? fv.fp_n : 1;
     const int fp_shift = kOver ? fv.fp_shift : 0;
@@ -498,8 +561,9 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         ty = cl_fract(vp - 0.5f, &base_row);
         return (int)base_row;
     };
-    auto stage_rows = [&](int q) {
+    auto stage_rows = [&](int q, int set) {
         const ETN_GLOBAL FusedBp& bp = pass->bp[q];
+        float4* dst = recs + set * set_len;
         const ETN_GLOBAL float* res = global_ptr(bp.residual);
         const ETN_GLOBAL float* len = global_ptr(bp.length);
         // record e = (p * rowsets + ys) * row_len + k: plane p, pattern row ys, column k
@@ -526,11 +590,11 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
             }
             if (kFast) {
                 // R(tx) = A + B tx with A, B the row pair interpolated at ty
-                recs[e] = make_float4(fmaf(v[2] - v[0], ty, v[0]), fmaf(v[3] - v[1], ty, v[1]),
-                                      fmaf(v[6] - v[4], ty, v[4]), fmaf(v[7] - v[5], ty, v[5]));
+                dst[e] = make_float4(fmaf(v[2] - v[0], ty, v[0]), fmaf(v[3] - v[1], ty, v[1]),
+                                     fmaf(v[6] - v[4], ty, v[4]), fmaf(v[7] - v[5], ty, v[5]));
             } else {
-                recs[2 * e] = make_float4(v[0], v[1], v[2], v[3]);
-                recs[2 * e + 1] = make_float4(v[4], v[5], v[6], v[7]);
+                dst[2 * e] = make_float4(v[0], v[1], v[2], v[3]);
+                dst[2 * e + 1] = make_float4(v[4], v[5], v[6], v[7]);
             }
         }
         if (tid < np * rowsets) {
@@ -538,7 +602,7 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
             const float dy = kOver ? bp.samples[ys * bp.per_dir].y : bp.sample.y;
             float py, ty;
             plane_v(bp, p, dy, py, ty);
-            pterm[tid] = make_float2(bp.m[4] * (py + dy), ty);
+            pterm[set * np * rowsets + tid] = make_float2(bp.m[4] * (py + dy), ty);
         }
     };
 
@@ -583,230 +647,256 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         }
     }
     prefetch(z0, z1, 0, kPre);
-    if (nbp == 1) stage_rows(0);     // made visible by the first band's barrier
+    if constexpr (kRes != 0) {
+        // every back projection's records, before the first band needs them
+        if (nbp > 0) {
+            for (int q = 0; q < nbp; ++q) stage_rows(q, q);
+            __syncthreads();
+        }
+    } else if (nbp == 1) {
+        stage_rows(0, 0);            // made visible by the first band's barrier
+    }
+    // register back projection: the host launches it only where kThreads is a
+    // multiple of the groups per band row, so that a thread's groups keep one
+    // x in every band row; the voxel-centre x of the thread's 4 voxels, their
+    // plane and first x; group k's band row rz0 + k rdz
+    [[maybe_unused]] int rp = 0, rx0 = 0;
+    [[maybe_unused]] const int rz0 = tid / gpr, rdz = kThreads / gpr;
+    [[maybe_unused]] float xc[kRes ? 4 : 1];
+    if constexpr (kRes != 0) {
+        const int xw = (tid % gpr) * 4;
+        if (np > 1) {
+            rp = xw / nx;
+            rx0 = xw - rp * nx;
+        } else {
+            rx0 = xw;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xc[i] = s.bmin[0] + ((float)(rx0 + i) + 0.5f) * s.vs[0];
+    }
 
     for (int bi = 0; bi < nbands; ++bi) {
         band_range(bi, z0, z1);
         const int ng = (z1 - z0) * gpr;
-        // 1. prefetched band -> LDS
-        if (kVec) {
+        // the next band (loaded in two halves with f32 storage: see below)
+        constexpr int kSplit = sizeof(typename S::T) == 4 ? (kPre + 1) / 2 : kPre;
+        int nb0 = 0, nb1 = 0;
+        if (bi + 1 < nbands) band_range(bi + 1, nb0, nb1);
+        if constexpr (kRes != 0) {
+            // 1. the prefetched band with the previous block's back projections
+            //    applied in registers, then to LDS and HBM
+            float v[kPre][4];
+            float mw[kMask ? kPre : 1][4];
 #pragma unroll
             for (int k = 0; k < kPre; ++k) {
                 const int gi = k * kThreads + tid;
                 if (gi < ng) {
-                    float v[G];
-                    IO::unpack(pre[k], v);
-                    const int zz = gi / gpr, xg = gi - zz * gpr;
-                    // one 16-byte LDS store (width and xg * G are multiples of 4)
-                    *reinterpret_cast<float4*>(band + zz * width + xg * G) = make_float4(v[0], v[1], v[2], v[3]);
+                    IO::unpack(pre[k], v[k]);
                     if (kMask) {
                         const uint32_t m = mpre[k];
-                        *reinterpret_cast<float4*>(mband + zz * width + xg * G) =
-                            make_float4(weight(m & 255u), weight((m >> 8) & 255u), weight((m >> 16) & 255u),
-                                        weight(m >> 24));
+                        mw[k][0] = weight(m & 255u);
+                        mw[k][1] = weight((m >> 8) & 255u);
+                        mw[k][2] = weight((m >> 16) & 255u);
+                        mw[k][3] = weight(m >> 24);
                     }
                 }
             }
+            // voxel-centre z of group k's row
+            auto zc = [&](int k) { return s.bmin[2] + ((float)(z0 + rz0 + k * rdz) + 0.5f) * s.vs[2]; };
+            if (bi + 1 < nbands) prefetch(nb0, nb1, 0, kSplit);
+            // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the
+            // host), px and pz the voxel centre plus the sample offset
+#pragma unroll 1
+            for (int q = 0; q < nbp; ++q) {
+                const ETN_GLOBAL FusedBp& bp = pass->bp[q];
+                const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12];
+                const float4 smp = bp.sample;
+                const float lc = bp.lambda * smp.w;
+                const float2 pt = pterm[q * np + rp];
+                const float4* pr = recs + q * set_len + rp * row_len;
+                float xt[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) xt[i] = m0 * (xc[i] + smp.x) + pt.x;
+#pragma unroll
+                for (int k = 0; k < kPre; ++k) {
+                    if (k * kThreads + tid < ng) {
+                        const float zt = m8 * (zc(k) + smp.z);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            v[k][i] = bp_update_fast<S, kMask, false>(pr, pad_l, xt[i], zt, m12, lc, v[k][i],
+                                                                      kMask ? mw[k][i] : 1.0f, 0u, 0u);
+                    }
+                    // one group's record reads in flight at a time: hoisting all
+                    // of them would take 4 VGPRs per voxel
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPre; ++k) {
+                if (k * kThreads + tid < ng) {
+                    const int zz = rz0 + k * rdz, xw = (tid % gpr) * 4;
+                    *reinterpret_cast<float4*>(band + zz * width + xw) = make_float4(v[k][0], v[k][1], v[k][2], v[k][3]);
+                    if (kMask)
+                        *reinterpret_cast<float4*>(mband + zz * width + xw) =
+                            make_float4(mw[k][0], mw[k][1], mw[k][2], mw[k][3]);
+                    if (nbp > 0) IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, v[k]);
+                }
+            }
+            __syncthreads();
         } else {
-            for (int gi = tid; gi < ng; gi += kThreads) {
-                const int zz = gi / gpr, xg = gi - zz * gpr;
-                const size_t at = (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G;
-                float v[G];
-                IO::unpack(IO::load(vol, at), v);
+            // 1. prefetched band -> LDS
+            if (kVec) {
 #pragma unroll
-                for (int i = 0; i < G; ++i) {
-                    band[zz * width + xg * G + i] = v[i];
-                    if (kMask) mband[zz * width + xg * G + i] = mask ? weight(mask[at + i]) : 1.0f;
-                }
-            }
-        }
-        // the prefetch registers are free again: start fetching the next band
-        // before waiting for the other waves (it only reads planes this
-        // workgroup stores later).  With f32 storage half of the groups go
-        // now and the other half before the forward projections, so that
-        // memory requests stay in flight through both phases of the band (the
-        // f32 launch is bound by how well memory overlaps the arithmetic:
-        // 568 -> 612 GUPS at 512^3).  16-bit storage moves half the bytes and
-        // is bound by VALU issue; it loads the whole band now (split, it lost 4%).
-        constexpr int kSplit = sizeof(typename S::T) == 4 ? (kPre + 1) / 2 : kPre;
-        int nb0 = 0, nb1 = 0;
-        if (bi + 1 < nbands) {
-            band_range(bi + 1, nb0, nb1);
-            prefetch(nb0, nb1, 0, kSplit);
-        }
-        __syncthreads();
-
-        // 2. back projections of the previous block, in block order; each thread
-        //    updates (and finally stores) the groups it staged
-        for (int q = 0; q < nbp; ++q) {
-            if (nbp > 1) {
-                stage_rows(q);
-                __syncthreads();
-            }
-            // the pass lives in global memory: copy what the loops use into registers
-            const ETN_GLOBAL FusedBp& bp = pass->bp[q];
-            const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda;
-            const float4 smp = bp.sample;
-            const float chord = smp.w, lc = lam * chord;
-            const uint32_t salt = kSR ? bp.sr_salt : 0u;
-            // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
-            auto x_term = [&](int x, float a4) {
-                const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + smp.x;
-                return m0 * px + a4;
-            };
-            auto z_term = [&](int zz) {
-                const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + smp.z;
-                return m8 * pz;
-            };
-            // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
-            // vid: the voxel's index in the volume (stochastic rounding's random bits)
-            auto update = [&](const float4* pr, float ty, float xt, float zt, float value, float mv, uint32_t vid) {
-                if (kFast) {
-                    // u in the reference's order (u is a few hundred pixels, so a
-                    // reordered sum would move the weights by ~1e-4); fract without
-                    // the clamp below 1 (at tx == 1 the interpolation gives the next
-                    // column's value, its limit).  The staged rows are padded (on the
-                    // host) beyond every column the member's projections reach, so the
-                    // index needs no clamp.
-                    float u = xt + zt;
-                    u = u + m12 * 1.0f;
-                    u = u - 0.5f;
-                    const float bu = floorf(u);
-                    const float tx = u - bu;
-                    const int i = (int)bu + pad_l;
-                    const float4 c = pr[i];
-                    const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
-                    const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
-                    const float next = kMask ? fmaf(mv, contrib, value) : value + contrib;
-                    value = (L > kEpsilon) ? next : value;
-                    if constexpr (kSR) return S::round_stochastic(value, sr_hash(vid, salt, value));
-                    return S::round(value);
-                }
-                float u = xt + zt;
-                u = u + m12 * 1.0f;
-                float bu;
-                const float tx = cl_fract(u - 0.5f, &bu);
-                const int i = padded_index((int)bu, nu);
-                const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
-                const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
-                const float L = L0 + (L1 - L0) * ty;
-                const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
-                const float Rv = R0 + (R1 - R0) * ty;
-                const float wgt = chord / L;
-                const float contrib = Rv * lam * wgt;
-                const float sum = 0.0f + contrib;             // sum over the one sample
-                const float add = kMask ? mv * sum : sum;     // weight * sum / 1
-                value = (L > kEpsilon) ? value + add : value; // used == 1 exactly when L > eps
-                return S::round(value);
-            };
-            const int nrow = z1 - z0;
-            if constexpr (kOver) {
-                // the sample loop of ParallelBeamsBackProjection.cl:26-52 over the
-                // pattern; pattern row ys reads its own staged row pair
-                const int nsmp = bp.nsamples, pd = bp.per_dir;
-#pragma unroll 1
-                for (int gi = tid; gi < ng; gi += kThreads) {
-                    const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
-                    int p = 0, x0 = xw;
-                    if (np > 1) {
-                        p = xw / nx;
-                        x0 = xw - p * nx;
-                    }
-                    const float cz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2];
-                    float out[G];
-#pragma unroll
-                    for (int i = 0; i < G; ++i) {
-                        const float cx = s.bmin[0] + ((float)(x0 + i) + 0.5f) * s.vs[0];
-                        float sum = 0.0f;
-                        int used = 0;
-#pragma unroll 1
-                        for (int t = 0; t < nsmp; ++t) {
-                            const float4 sm = bp.samples[t];
-                            const int ys = t / pd;
-                            const float2 pt = pterm[p * rowsets + ys];
-                            float u = m0 * (cx + sm.x) + pt.x;
-                            u = u + m8 * (cz + sm.z);
-                            u = u + m12 * 1.0f;
-                            float L, contrib;
-                            if (kFast) {
-                                u = u - 0.5f;
-                                const float bu = floorf(u);
-                                const float tx = u - bu;
-                                const float4 c = recs[(p * rowsets + ys) * row_len + (int)bu + pad_l];
-                                const float Rv = fmaf(c.y, tx, c.x);
-                                L = fmaf(c.w, tx, c.z);
-                                contrib = Rv * (lam * sm.w) * __builtin_amdgcn_rcpf(L);
-                            } else {
-                                // bp_voxel_sum's bilinear lookups and weight, operation for operation
-                                float bu;
-                                const float tx = cl_fract(u - 0.5f, &bu);
-                                const int i = padded_index((int)bu, nu);
-                                const float4* pr = recs + (p * rowsets + ys) * 2 * row_len;
-                                const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
-                                const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
-                                L = L0 + (L1 - L0) * pt.y;
-                                const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
-                                const float Rv = R0 + (R1 - R0) * pt.y;
-                                const float wgt = sm.w / L;
-                                contrib = Rv * lam * wgt;
-                            }
-                            const bool ok = L > kEpsilon;
-                            sum += ok ? contrib : 0.0f;
-                            used += ok ? 1 : 0;
-                        }
-                        const int at = zz * width + xw + i;
-                        float value = band[at];
-                        if (used != 0) value += (kMask ? band[at + kMaskOff] * sum : sum) / (float)used;
-                        out[i] = S::round(value);
-                        band[at] = out[i];
-                    }
-                    if (q == nbp - 1)
-                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
-                }
-            } else if (kThreads % gpr == 0) {
-                // every thread keeps the same x group (and plane) in every band row
-                const int xw = (tid % gpr) * G, dz = kThreads / gpr;
-                int p = 0, x0 = xw;
-                if (np > 1) {
-                    p = xw / nx;
-                    x0 = xw - p * nx;
-                }
-                const float2 pt = pterm[p];
-                const float4* pr = recs + p * kRecStride * row_len;
-                float xt[G];
-#pragma unroll
-                for (int i = 0; i < G; ++i) xt[i] = x_term(x0 + i, pt.x);
-#pragma unroll 1
-                for (int zz = tid / gpr; zz < nrow; zz += dz) {
-                    const float zt = z_term(zz);
-                    float out[G];
-                    float* cell = band + zz * width + xw;
-                    const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
-                    if constexpr (G == 4) {
-                        // four voxels, 16-byte aligned (width and xw are multiples of 4)
-                        const float4 in = *reinterpret_cast<const float4*>(cell);
-                        const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskOff)
-                                                : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-                        out[0] = update(pr, pt.y, xt[0], zt, in.x, mk.x, vid);
-                        out[1] = update(pr, pt.y, xt[1], zt, in.y, mk.y, vid + 1u);
-                        out[2] = update(pr, pt.y, xt[2], zt, in.z, mk.z, vid + 2u);
-                        out[3] = update(pr, pt.y, xt[3], zt, in.w, mk.w, vid + 3u);
-                        *reinterpret_cast<float4*>(cell) = make_float4(out[0], out[1], out[2], out[3]);
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < G; ++i) {
-                            out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskOff] : 1.0f,
-                                            vid + (uint32_t)i);
-                            cell[i] = out[i];
+                for (int k = 0; k < kPre; ++k) {
+                    const int gi = k * kThreads + tid;
+                    if (gi < ng) {
+                        float v[G];
+                        IO::unpack(pre[k], v);
+                        const int zz = gi / gpr, xg = gi - zz * gpr;
+                        // one 16-byte LDS store (width and xg * G are multiples of 4)
+                        *reinterpret_cast<float4*>(band + zz * width + xg * G) = make_float4(v[0], v[1], v[2], v[3]);
+                        if (kMask) {
+                            const uint32_t m = mpre[k];
+                            *reinterpret_cast<float4*>(mband + zz * width + xg * G) =
+                                make_float4(weight(m & 255u), weight((m >> 8) & 255u), weight((m >> 16) & 255u),
+                                            weight(m >> 24));
                         }
                     }
-                    if (q == nbp - 1)
-                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
                 }
             } else {
-#pragma unroll 1
                 for (int gi = tid; gi < ng; gi += kThreads) {
-                    const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
+                    const int zz = gi / gpr, xg = gi - zz * gpr;
+                    const size_t at = (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xg * G;
+                    float v[G];
+                    IO::unpack(IO::load(vol, at), v);
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        band[zz * width + xg * G + i] = v[i];
+                        if (kMask) mband[zz * width + xg * G + i] = mask ? weight(mask[at + i]) : 1.0f;
+                    }
+                }
+            }
+            // the prefetch registers are free again: start fetching the next band
+            // before waiting for the other waves (it only reads planes this
+            // workgroup stores later).  With f32 storage half of the groups go
+            // now and the other half before the forward projections, so that
+            // memory requests stay in flight through both phases of the band (the
+            // f32 launch is bound by how well memory overlaps the arithmetic:
+            // 568 -> 612 GUPS at 512^3).  16-bit storage moves half the bytes and
+            // is bound by VALU issue; it loads the whole band now (split, it lost 4%).
+            if (bi + 1 < nbands) prefetch(nb0, nb1, 0, kSplit);
+            __syncthreads();
+
+            // 2. back projections of the previous block, in block order; each thread
+            //    updates (and finally stores) the groups it staged
+            for (int q = 0; q < nbp; ++q) {
+                if (nbp > 1) {
+                    stage_rows(q, 0);
+                    __syncthreads();
+                }
+                // the pass lives in global memory: copy what the loops use into registers
+                const ETN_GLOBAL FusedBp& bp = pass->bp[q];
+                const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda;
+                const float4 smp = bp.sample;
+                const float chord = smp.w, lc = lam * chord;
+                const uint32_t salt = kSR ? bp.sr_salt : 0u;
+                // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
+                auto x_term = [&](int x, float a4) {
+                    const float px = s.bmin[0] + ((float)x + 0.5f) * s.vs[0] + smp.x;
+                    return m0 * px + a4;
+                };
+                auto z_term = [&](int zz) {
+                    const float pz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2] + smp.z;
+                    return m8 * pz;
+                };
+                // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
+                // vid: the voxel's index in the volume (stochastic rounding's random bits)
+                auto update = [&](const float4* pr, float ty, float xt, float zt, float value, float mv, uint32_t vid) {
+                    if (kFast) return bp_update_fast<S, kMask, kSR>(pr, pad_l, xt, zt, m12, lc, value, mv, vid, salt);
+                    float u = xt + zt;
+                    u = u + m12 * 1.0f;
+                    float bu;
+                    const float tx = cl_fract(u - 0.5f, &bu);
+                    const int i = padded_index((int)bu, nu);
+                    const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
+                    const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
+                    const float L = L0 + (L1 - L0) * ty;
+                    const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
+                    const float Rv = R0 + (R1 - R0) * ty;
+                    const float wgt = chord / L;
+                    const float contrib = Rv * lam * wgt;
+                    const float sum = 0.0f + contrib;             // sum over the one sample
+                    const float add = kMask ? mv * sum : sum;     // weight * sum / 1
+                    value = (L > kEpsilon) ? value + add : value; // used == 1 exactly when L > eps
+                    return S::round(value);
+                };
+                const int nrow = z1 - z0;
+                if constexpr (kOver) {
+                    // the sample loop of ParallelBeamsBackProjection.cl:26-52 over the
+                    // pattern; pattern row ys reads its own staged row pair
+                    const int nsmp = bp.nsamples, pd = bp.per_dir;
+#pragma unroll 1
+                    for (int gi = tid; gi < ng; gi += kThreads) {
+                        const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
+                        int p = 0, x0 = xw;
+                        if (np > 1) {
+                            p = xw / nx;
+                            x0 = xw - p * nx;
+                        }
+                        const float cz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2];
+                        float out[G];
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const float cx = s.bmin[0] + ((float)(x0 + i) + 0.5f) * s.vs[0];
+                            float sum = 0.0f;
+                            int used = 0;
+#pragma unroll 1
+                            for (int t = 0; t < nsmp; ++t) {
+                                const float4 sm = bp.samples[t];
+                                const int ys = t / pd;
+                                const float2 pt = pterm[p * rowsets + ys];
+                                float u = m0 * (cx + sm.x) + pt.x;
+                                u = u + m8 * (cz + sm.z);
+                                u = u + m12 * 1.0f;
+                                float L, contrib;
+                                if (kFast) {
+                                    u = u - 0.5f;
+                                    const float bu = floorf(u);
+                                    const float tx = u - bu;
+                                    const float4 c = recs[(p * rowsets + ys) * row_len + (int)bu + pad_l];
+                                    const float Rv = fmaf(c.y, tx, c.x);
+                                    L = fmaf(c.w, tx, c.z);
+                                    contrib = Rv * (lam * sm.w) * __builtin_amdgcn_rcpf(L);
+                                } else {
+                                    // bp_voxel_sum's bilinear lookups and weight, operation for operation
+                                    float bu;
+                                    const float tx = cl_fract(u - 0.5f, &bu);
+                                    const int i = padded_index((int)bu, nu);
+                                    const float4* pr = recs + (p * rowsets + ys) * 2 * row_len;
+                                    const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
+                                    const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
+                                    L = L0 + (L1 - L0) * pt.y;
+                                    const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
+                                    const float Rv = R0 + (R1 - R0) * pt.y;
+                                    const float wgt = sm.w / L;
+                                    contrib = Rv * lam * wgt;
+                                }
+                                const bool ok = L > kEpsilon;
+                                sum += ok ? contrib : 0.0f;
+                                used += ok ? 1 : 0;
+                            }
+                            const int at = zz * width + xw + i;
+                            float value = band[at];
+                            if (used != 0) value += (kMask ? band[at + kMaskOff] * sum : sum) / (float)used;
+                            out[i] = S::round(value);
+                            band[at] = out[i];
+                        }
+                        if (q == nbp - 1)
+                            IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
+                    }
+                } else if (kThreads % gpr == 0) {
+                    // every thread keeps the same x group (and plane) in every band row
+                    const int xw = (tid % gpr) * G, dz = kThreads / gpr;
                     int p = 0, x0 = xw;
                     if (np > 1) {
                         p = xw / nx;
@@ -814,23 +904,65 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
                     }
                     const float2 pt = pterm[p];
                     const float4* pr = recs + p * kRecStride * row_len;
-                    const float zt = z_term(zz);
-                    float out[G];
-                    const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
+                    float xt[G];
 #pragma unroll
-                    for (int i = 0; i < G; ++i) {
-                        const int c = zz * width + xw + i;
-                        out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskOff] : 1.0f,
-                                        vid + (uint32_t)i);
-                        band[c] = out[i];
+                    for (int i = 0; i < G; ++i) xt[i] = x_term(x0 + i, pt.x);
+#pragma unroll 1
+                    for (int zz = tid / gpr; zz < nrow; zz += dz) {
+                        const float zt = z_term(zz);
+                        float out[G];
+                        float* cell = band + zz * width + xw;
+                        const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
+                        if constexpr (G == 4) {
+                            // four voxels, 16-byte aligned (width and xw are multiples of 4)
+                            const float4 in = *reinterpret_cast<const float4*>(cell);
+                            const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskOff)
+                                                    : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+                            out[0] = update(pr, pt.y, xt[0], zt, in.x, mk.x, vid);
+                            out[1] = update(pr, pt.y, xt[1], zt, in.y, mk.y, vid + 1u);
+                            out[2] = update(pr, pt.y, xt[2], zt, in.z, mk.z, vid + 2u);
+                            out[3] = update(pr, pt.y, xt[3], zt, in.w, mk.w, vid + 3u);
+                            *reinterpret_cast<float4*>(cell) = make_float4(out[0], out[1], out[2], out[3]);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < G; ++i) {
+                                out[i] = update(pr, pt.y, xt[i], zt, cell[i], kMask ? cell[i + kMaskOff] : 1.0f,
+                                                vid + (uint32_t)i);
+                                cell[i] = out[i];
+                            }
+                        }
+                        if (q == nbp - 1)
+                            IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
                     }
-                    if (q == nbp - 1)
-                        IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
+                } else {
+#pragma unroll 1
+                    for (int gi = tid; gi < ng; gi += kThreads) {
+                        const int zz = gi / gpr, xw = (gi - zz * gpr) * G;
+                        int p = 0, x0 = xw;
+                        if (np > 1) {
+                            p = xw / nx;
+                            x0 = xw - p * nx;
+                        }
+                        const float2 pt = pterm[p];
+                        const float4* pr = recs + p * kRecStride * row_len;
+                        const float zt = z_term(zz);
+                        float out[G];
+                        const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const int c = zz * width + xw + i;
+                            out[i] = update(pr, pt.y, x_term(x0 + i, pt.x), zt, band[c], kMask ? band[c + kMaskOff] : 1.0f,
+                                            vid + (uint32_t)i);
+                            band[c] = out[i];
+                        }
+                        if (q == nbp - 1)
+                            IO::store(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw, out);
+                    }
                 }
+                if (nbp > 1) __syncthreads();
             }
-            if (nbp > 1) __syncthreads();
+            if (nbp == 1) __syncthreads();
         }
-        if (nbp == 1) __syncthreads();
 
         if (bi + 1 < nbands) prefetch(nb0, nb1, kSplit, kPre);
 
@@ -927,17 +1059,30 @@ __global__ void __launch_bounds__(kThreads, RPT <= 2 ? 4 : 1)
         }
 }
 
-template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, bool kSR = false>
+template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, bool kSR = false,
+          int kRes = 0>
 void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
 {
-    if (l.lds > sizeof(float) * (size_t)FusedShape<kThreads, (int)sizeof(typename S::T)>::kLds)
-        fail("fused plane kernel: tile exceeds the LDS array");
+    using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes == 2>::type;
+    if (l.lds > sizeof(float) * (size_t)Shape::kLds) fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kThreads);
     switch (l.rpt) {
-        case 1: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 2: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        case 4: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
-        default: fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8, kSR><<<grid, block, 0, stream>>>(l.vols, l.work, l.passes); break;
+        case 1:
+            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1, kSR, kRes>
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+            break;
+        case 2:
+            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2, kSR, kRes>
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+            break;
+        case 4:
+            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4, kSR, kRes>
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+            break;
+        default:
+            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8, kSR, kRes>
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+            break;
     }
 }
 
@@ -961,6 +1106,26 @@ void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 template <typename S, bool kFast, bool kMask, bool kOver>
 void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
+    if (l.res != 0) {
+        if constexpr (kFast && !kOver) {
+            if (vec && !cap && !l.sr) {
+                if (l.res == 2 && l.threads == 512) {
+                    launch_fused_shape<S, true, false, true, kMask, false, 512, false, 2>(l, stream);
+                    return;
+                }
+                if (l.res == 1 && l.threads == 512) {
+                    launch_fused_shape<S, true, false, true, kMask, false, 512, false, 1>(l, stream);
+                    return;
+                }
+                if (l.res == 1 && l.threads == 256) {
+                    launch_fused_shape<S, true, false, true, kMask, false, 256, false, 1>(l, stream);
+                    return;
+                }
+            }
+        }
+        fail("fused plane kernel: the register back projection needs fast arithmetic, nx a multiple of 4, no "
+             "oversampling, stochastic rounding or step limit, and 512 threads for the resident record sets");
+    }
     if (l.sr) {
         // stochastic rounding: 16-bit storage on the fast kernels without a
         // mask, oversampling or the step limit, nx a multiple of 4

@@ -95,7 +95,7 @@ WeightedBackProjection::WeightedBackProjection(const etn_reconstruction_desc& d,
     ETN_HIP(hipStreamSynchronize(stream_));
     if ((d.flags & (ETN_RECON_FLAG_GENERIC_KERNELS | ETN_RECON_FLAG_EXACT_ARITHMETIC)) == 0)
         fused_ = FusedBackProjector::create(volume_.get(), geometries_, nu, nv, d.bp_samples ? d.bp_samples : 1,
-                                            stream_);
+                                            (d.flags & ETN_RECON_FLAG_LDS_BACK_PROJECTION) != 0, stream_);
 }
 
 WeightedBackProjection::~WeightedBackProjection()

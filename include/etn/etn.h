@@ -194,11 +194,16 @@ enum {
                                                    nearest); needs the fast fused kernels: f16 or bf16 storage,
                                                    no mask, no oversampling, nx a multiple of 4, nx + nz <= 2049;
                                                    creation fails otherwise */,
-    ETN_RECON_FLAG_16BIT_PROJECTIONS = 1u << 5   /* the measured projections are stored in the volume's 16-bit
+    ETN_RECON_FLAG_16BIT_PROJECTIONS = 1u << 5,  /* the measured projections are stored in the volume's 16-bit
                                                    type (voxel_type f16 or bf16): `projections` addresses
                                                    projection_count * nu * nv such values.  Arithmetic stays
                                                    FP32.  Extension (the reference keeps projections in f32);
                                                    not for "wbp" */
+    ETN_RECON_FLAG_LDS_BACK_PROJECTION = 1u << 6 /* fast fused kernels: apply the back projections to the band in
+                                                   LDS, one projection after another, instead of in registers as
+                                                   the band arrives from HBM (the default where it applies: nx a
+                                                   multiple of 4, no oversampling).  Same volumes bit for bit;
+                                                   for comparison and measurement */
 };
 
 /* Parameters of BlockIterativeReconstructionOperator, named after the
