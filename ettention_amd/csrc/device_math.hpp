@@ -24,15 +24,26 @@ constexpr float kEpsilon = 1e-4f;     // ocl_src/Math.hl:4
 // VGPRs); marking it global gives global loads and scalar loads of uniform data.
 // (The host pass of the single-source compile only parses device code: no
 // address space there.)
+// Marking a table constant (it is not written while the kernel runs) turns
+// its uniform reads into scalar loads even after the kernel's own stores:
+// a vector load of a per-projection parameter would otherwise wait (vmcnt, in
+// issue order) for every band load issued before it.
 #ifdef __HIP_DEVICE_COMPILE__
 #define ETN_GLOBAL __attribute__((address_space(1)))
+#define ETN_CONST __attribute__((address_space(4)))
 #else
 #define ETN_GLOBAL
+#define ETN_CONST
 #endif
 template <typename T>
 __device__ __forceinline__ ETN_GLOBAL T* global_ptr(T* p)
 {
     return (ETN_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const ETN_CONST T* const_ptr(const ETN_GLOBAL T* p)
+{
+    return (const ETN_CONST T*)(const T*)p;
 }
 
 // ---- voxel storage ---------------------------------------------------------

@@ -374,24 +374,31 @@ __device__ __forceinline__ float lerp_pair(float a, float d, float t) { return a
 // One single-sample back projection update of a voxel with fast arithmetic
 // (ParallelBeamsBackProjection.cl:26-52): xt + zt = m0 px + m4 py + m8 pz, u
 // in the reference's order (u is a few hundred pixels, so a reordered sum
-// would move the weights by ~1e-4); fract without the clamp below 1 (at tx == 1
-// the interpolation gives the next column's value, its limit).  pr: the
-// plane's staged records, padded (on the host) beyond every column the
-// member's projections reach, so the index needs no clamp.  lc: lambda times
-// the in-voxel chord; mv: the mask weight; vid: the voxel's index in the
-// volume (stochastic rounding's random bits).
+// would move the weights by ~1e-4).  The bilinear base column and weight come
+// from one floor-convert and one fract (fract clamps below 1: a tx that
+// u - floor(u) would round up to 1 stays 1 - 2^-24, the limit the
+// interpolation approaches either way).  pr: the plane's staged records from
+// column -pad_l on, padded (on the host) beyond every column the member's
+// projections reach, so the index needs no clamp, with the residual pair
+// scaled by lambda times the in-voxel chord at staging.  mv: the mask weight;
+// vid: the voxel's index in the volume (stochastic rounding's random bits).
+__device__ __forceinline__ int floor_to_int(float u)
+{
+    int i;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(u));
+    return i;
+}
 template <typename S, bool kMask, bool kSR>
-__device__ __forceinline__ float bp_update_fast(const float4* pr, int pad_l, float xt, float zt, float m12, float lc,
-                                                float value, float mv, uint32_t vid, uint32_t salt)
+__device__ __forceinline__ float bp_update_fast(const float4* pr, float xt, float zt, float m12, float value, float mv,
+                                                uint32_t vid, uint32_t salt)
 {
     float u = xt + zt;
     u = u + m12 * 1.0f;
     u = u - 0.5f;
-    const float bu = floorf(u);
-    const float tx = u - bu;
-    const float4 c = pr[(int)bu + pad_l];
+    const float tx = __builtin_amdgcn_fractf(u);
+    const float4 c = pr[floor_to_int(u)];
     const float Rv = fmaf(c.y, tx, c.x), L = fmaf(c.w, tx, c.z);
-    const float contrib = Rv * lc * __builtin_amdgcn_rcpf(L);
+    const float contrib = Rv * __builtin_amdgcn_rcpf(L);
     const float next = kMask ? fmaf(mv, contrib, value) : value + contrib;
     value = (L > kEpsilon) ? next : value;
     if constexpr (kSR) return S::round_stochastic(value, sr_hash(vid, salt, value));
@@ -589,8 +596,10 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                 v[4 + 2 * rr + 1] = l1 - l0;
             }
             if (kFast) {
-                // R(tx) = A + B tx with A, B the row pair interpolated at ty
-                dst[e] = make_float4(fmaf(v[2] - v[0], ty, v[0]), fmaf(v[3] - v[1], ty, v[1]),
+                // R(tx) = A + B tx with A, B the row pair interpolated at ty; a
+                // single sample's A and B carry the weight lambda * chord
+                const float lc = kOver ? 1.0f : bp.lambda * bp.sample.w;
+                dst[e] = make_float4(lc * fmaf(v[2] - v[0], ty, v[0]), lc * fmaf(v[3] - v[1], ty, v[1]),
                                      fmaf(v[6] - v[4], ty, v[4]), fmaf(v[7] - v[5], ty, v[5]));
             } else {
                 dst[2 * e] = make_float4(v[0], v[1], v[2], v[3]);
@@ -708,12 +717,11 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
             // host), px and pz the voxel centre plus the sample offset
 #pragma unroll 1
             for (int q = 0; q < nbp; ++q) {
-                const ETN_GLOBAL FusedBp& bp = pass->bp[q];
+                const ETN_CONST FusedBp& bp = const_ptr(pass)->bp[q];
                 const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12];
                 const float4 smp = bp.sample;
-                const float lc = bp.lambda * smp.w;
                 const float2 pt = pterm[q * np + rp];
-                const float4* pr = recs + q * set_len + rp * row_len;
+                const float4* pr = recs + q * set_len + rp * row_len + pad_l;
                 float xt[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) xt[i] = m0 * (xc[i] + smp.x) + pt.x;
@@ -723,7 +731,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                         const float zt = m8 * (zc(k) + smp.z);
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
-                            v[k][i] = bp_update_fast<S, kMask, false>(pr, pad_l, xt[i], zt, m12, lc, v[k][i],
+                            v[k][i] = bp_update_fast<S, kMask, false>(pr, xt[i], zt, m12, v[k][i],
                                                                       kMask ? mw[k][i] : 1.0f, 0u, 0u);
                     }
                     // one group's record reads in flight at a time: hoisting all
@@ -794,11 +802,12 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                     stage_rows(q, 0);
                     __syncthreads();
                 }
-                // the pass lives in global memory: copy what the loops use into registers
-                const ETN_GLOBAL FusedBp& bp = pass->bp[q];
+                // the pass lives in global memory: copy what the loops use into
+                // registers (scalar loads: the next band's loads are in flight)
+                const ETN_CONST FusedBp& bp = const_ptr(pass)->bp[q];
                 const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12], lam = bp.lambda;
                 const float4 smp = bp.sample;
-                const float chord = smp.w, lc = lam * chord;
+                const float chord = smp.w;
                 const uint32_t salt = kSR ? bp.sr_salt : 0u;
                 // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the host)
                 auto x_term = [&](int x, float a4) {
@@ -812,7 +821,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                 // ParallelBeamsBackProjection.cl:26-52 for one sample; pr: the plane's records
                 // vid: the voxel's index in the volume (stochastic rounding's random bits)
                 auto update = [&](const float4* pr, float ty, float xt, float zt, float value, float mv, uint32_t vid) {
-                    if (kFast) return bp_update_fast<S, kMask, kSR>(pr, pad_l, xt, zt, m12, lc, value, mv, vid, salt);
+                    if (kFast) return bp_update_fast<S, kMask, kSR>(pr + pad_l, xt, zt, m12, value, mv, vid, salt);
                     float u = xt + zt;
                     u = u + m12 * 1.0f;
                     float bu;
