@@ -671,6 +671,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
     // plane and first x; group k's band row rz0 + k rdz
     [[maybe_unused]] int rp = 0, rx0 = 0;
     [[maybe_unused]] const int rz0 = tid / gpr, rdz = kThreads / gpr;
+    [[maybe_unused]] const float4* rrec = recs;   // the plane's records of set 0 from column -pad_l on
     [[maybe_unused]] float xc[kRes ? 4 : 1];
     if constexpr (kRes != 0) {
         const int xw = (tid % gpr) * 4;
@@ -682,6 +683,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) xc[i] = s.bmin[0] + ((float)(rx0 + i) + 0.5f) * s.vs[0];
+        rrec = recs + rp * row_len + pad_l;
     }
 
     for (int bi = 0; bi < nbands; ++bi) {
@@ -710,8 +712,10 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                     }
                 }
             }
-            // voxel-centre z of group k's row
-            auto zc = [&](int k) { return s.bmin[2] + ((float)(z0 + rz0 + k * rdz) + 0.5f) * s.vs[2]; };
+            // voxel-centre z of group k's row (kept across the projections)
+            float zc[kPre];
+#pragma unroll
+            for (int k = 0; k < kPre; ++k) zc[k] = s.bmin[2] + ((float)(z0 + rz0 + k * rdz) + 0.5f) * s.vs[2];
             if (bi + 1 < nbands) prefetch(nb0, nb1, 0, kSplit);
             // u = ((m0 px + m4 py) + m8 pz) + m12 w, w == 1 exactly (checked on the
             // host), px and pz the voxel centre plus the sample offset
@@ -721,14 +725,19 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                 const float m0 = bp.m[0], m8 = bp.m[8], m12 = bp.m[12];
                 const float4 smp = bp.sample;
                 const float2 pt = pterm[q * np + rp];
-                const float4* pr = recs + q * set_len + rp * row_len + pad_l;
+                // set q's records: their LDS byte offset as one VGPR, so that a
+                // record address is a single shift-add of the column
+                uint32_t prb = (uint32_t)(reinterpret_cast<const char*>(rrec + q * set_len) -
+                                          reinterpret_cast<const char*>(lds));
+                asm("" : "+v"(prb));
+                const float4* pr = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(lds) + prb);
                 float xt[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) xt[i] = m0 * (xc[i] + smp.x) + pt.x;
 #pragma unroll
                 for (int k = 0; k < kPre; ++k) {
                     if (k * kThreads + tid < ng) {
-                        const float zt = m8 * (zc(k) + smp.z);
+                        const float zt = m8 * (zc[k] + smp.z);
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
                             v[k][i] = bp_update_fast<S, kMask, false>(pr, xt[i], zt, m12, v[k][i],
