@@ -98,6 +98,22 @@ struct F16 {   // vload_half / vstore_half (round to nearest even), HalfFloatSup
     }
     template <typename P>
     static __device__ __forceinline__ void store(P p, size_t i, float f) { p[i] = __float2half_rn(f); }
+    // two values in one v_cvt_pk_f16_f32 (gfx950, round to nearest even like
+    // __float2half_rn): a in the low half, b in the high half
+    typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ uint32_t pack2(float a, float b)
+    {
+        const f32x2 v = {a, b};
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2));
+    }
+    // the two halves of a packed pair, low half first (exact)
+    static __device__ __forceinline__ void unpack2(uint32_t h, float& a, float& b)
+    {
+        const f32x2 v = __builtin_convertvector(__builtin_bit_cast(f16x2, h), f32x2);
+        a = v.x;
+        b = v.y;
+    }
 };
 struct BF16 {  // bfloat16, round to nearest even
     using T = uint16_t;
@@ -112,6 +128,11 @@ struct BF16 {  // bfloat16, round to nearest even
     }
     static __device__ __forceinline__ uint16_t pack(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
     static __device__ __forceinline__ float unpack(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+    static __device__ __forceinline__ void unpack2(uint32_t h, float& a, float& b)
+    {
+        a = __uint_as_float(h << 16);
+        b = __uint_as_float(h & 0xffff0000u);
+    }
     template <typename P>
     static __device__ __forceinline__ float load(P p, size_t i) { return unpack(p[i]); }
     static __device__ __forceinline__ float round(float f) { return unpack(pack(f)); }

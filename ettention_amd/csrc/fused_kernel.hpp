@@ -328,21 +328,14 @@ struct GroupIo16 {   // 4 x 16-bit voxels as one 8-byte access
     }
     static __device__ __forceinline__ void unpack(const V& v, float* o)
     {
-        o[0] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 0);
-        o[1] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 1);
-        o[2] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 2);
-        o[3] = S16::load(reinterpret_cast<const typename S16::T*>(&v), 3);
+        S16::unpack2(v.x, o[0], o[1]);
+        S16::unpack2(v.y, o[2], o[3]);
     }
     static __device__ __forceinline__ void store(ETN_GLOBAL typename S16::T* p, size_t i, const float* f)
     {
         V v;
-        if constexpr (std::is_same<S16, BF16>::value) {
-            v.x = BF16::pack2(f[0], f[1]);
-            v.y = BF16::pack2(f[2], f[3]);
-        } else {
-            typename S16::T* t = reinterpret_cast<typename S16::T*>(&v);
-            for (int k = 0; k < 4; ++k) S16::store(t, k, f[k]);
-        }
+        v.x = S16::pack2(f[0], f[1]);
+        v.y = S16::pack2(f[2], f[3]);
         *reinterpret_cast<ETN_GLOBAL uint2*>(p + i) = v;
     }
 };
