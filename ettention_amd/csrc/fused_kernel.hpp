@@ -795,7 +795,11 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
             // 568 -> 612 GUPS at 512^3).  16-bit storage moves half the bytes and
             // is bound by VALU issue; it loads the whole band now (split, it lost 4%).
             if (bi + 1 < nbands) prefetch(nb0, nb1, 0, kSplit);
-            __syncthreads();
+            // every thread back-projects exactly the groups it committed, so
+            // with back projections the band needs no barrier here (the one
+            // after them makes it visible to the forward projections); the
+            // first band's makes the records staged before the loop visible
+            if (bi == 0 || nbp == 0) __syncthreads();
 
             // 2. back projections of the previous block, in block order; each thread
             //    updates (and finally stores) the groups it staged
