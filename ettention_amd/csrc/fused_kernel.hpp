@@ -338,6 +338,22 @@ struct GroupIo16 {   // 4 x 16-bit voxels as one 8-byte access
         v.y = S16::pack2(f[2], f[3]);
         *reinterpret_cast<ETN_GLOBAL uint2*>(p + i) = v;
     }
+    // the group rounded to storage (round to nearest even, as S16::round does
+    // value by value) in two packed conversions: the rounded values back in
+    // f, the packed group returned for store_packed
+    static __device__ __forceinline__ V round4(float* f)
+    {
+        V v;
+        v.x = S16::pack2(f[0], f[1]);
+        v.y = S16::pack2(f[2], f[3]);
+        S16::unpack2(v.x, f[0], f[1]);
+        S16::unpack2(v.y, f[2], f[3]);
+        return v;
+    }
+    static __device__ __forceinline__ void store_packed(ETN_GLOBAL typename S16::T* p, size_t i, const V& v)
+    {
+        *reinterpret_cast<ETN_GLOBAL uint2*>(p + i) = v;
+    }
 };
 template <>
 struct GroupIo<F16, true> : GroupIo16<F16> {};
@@ -381,7 +397,7 @@ __device__ __forceinline__ int floor_to_int(float u)
     asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(u));
     return i;
 }
-template <typename S, bool kMask, bool kSR>
+template <typename S, bool kMask, bool kSR, bool kRound = true>
 __device__ __forceinline__ float bp_update_fast(const float4* pr, float xt, float zt, float m12, float value, float mv,
                                                 uint32_t vid, uint32_t salt)
 {
@@ -394,6 +410,7 @@ __device__ __forceinline__ float bp_update_fast(const float4* pr, float xt, floa
     const float contrib = Rv * __builtin_amdgcn_rcpf(L);
     const float next = kMask ? fmaf(mv, contrib, value) : value + contrib;
     value = (L > kEpsilon) ? next : value;
+    if constexpr (!kRound) return value;   // the caller rounds (round4)
     if constexpr (kSR) return S::round_stochastic(value, sr_hash(vid, salt, value));
     return S::round(value);
 }
@@ -733,8 +750,10 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                         const float zt = m8 * (zc[k] + smp.z);
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
-                            v[k][i] = bp_update_fast<S, kMask, false>(pr, xt[i], zt, m12, v[k][i],
-                                                                      kMask ? mw[k][i] : 1.0f, 0u, 0u);
+                            v[k][i] = bp_update_fast<S, kMask, false, false>(pr, xt[i], zt, m12, v[k][i],
+                                                                             kMask ? mw[k][i] : 1.0f, 0u, 0u);
+                        // rounded to storage after every projection
+                        if constexpr (sizeof(typename S::T) == 2) IO::round4(v[k]);
                     }
                     // one group's record reads in flight at a time: hoisting all
                     // of them would take 4 VGPRs per voxel
@@ -928,7 +947,24 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
                         float out[G];
                         float* cell = band + zz * width + xw;
                         const uint32_t vid = (uint32_t)((size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw);
-                        if constexpr (G == 4) {
+                        if constexpr (G == 4 && kFast && !kSR && sizeof(typename S::T) == 2) {
+                            // 16-bit storage: the four updates rounded in two packed
+                            // conversions, the packed group stored as it is
+                            const float4 in = *reinterpret_cast<const float4*>(cell);
+                            const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskOff)
+                                                    : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+                            const float4* prp = pr + pad_l;
+                            out[0] = bp_update_fast<S, kMask, false, false>(prp, xt[0], zt, m12, in.x, mk.x, 0u, 0u);
+                            out[1] = bp_update_fast<S, kMask, false, false>(prp, xt[1], zt, m12, in.y, mk.y, 0u, 0u);
+                            out[2] = bp_update_fast<S, kMask, false, false>(prp, xt[2], zt, m12, in.z, mk.z, 0u, 0u);
+                            out[3] = bp_update_fast<S, kMask, false, false>(prp, xt[3], zt, m12, in.w, mk.w, 0u, 0u);
+                            const typename IO::V packed = IO::round4(out);
+                            *reinterpret_cast<float4*>(cell) = make_float4(out[0], out[1], out[2], out[3]);
+                            if (q == nbp - 1)
+                                IO::store_packed(vol, (size_t)(z0 + zz) * plane_stride + plane_base + (size_t)xw,
+                                                 packed);
+                            continue;
+                        } else if constexpr (G == 4) {
                             // four voxels, 16-byte aligned (width and xw are multiples of 4)
                             const float4 in = *reinterpret_cast<const float4*>(cell);
                             const float4 mk = kMask ? *reinterpret_cast<const float4*>(cell + kMaskOff)
