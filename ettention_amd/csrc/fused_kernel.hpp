@@ -448,7 +448,7 @@ __device__ __forceinline__ float bp_update_fast(const float4* pr, float xt, floa
 // roundings as on the LDS path, so the volumes are bit-identical.
 template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT, bool kSR,
           int kRes>
-__global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : RPT <= 2 ? 4 : 1)
+__global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver && RPT == 4)) ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes)
 {
