@@ -878,19 +878,29 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
                             x0 = xw - p * nx;
                         }
                         const float cz = s.bmin[2] + ((float)(z0 + zz) + 0.5f) * s.vs[2];
-                        float out[G];
+                        // samples outer, the group's voxels inner: each sample's
+                        // parameters and z term serve the G voxels, whose G
+                        // independent chains overlap their record reads; every
+                        // voxel still adds its samples in pattern order
+                        float cx[G], sum[G];
+                        int used[G];
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            const float cx = s.bmin[0] + ((float)(x0 + i) + 0.5f) * s.vs[0];
-                            float sum = 0.0f;
-                            int used = 0;
+                            cx[i] = s.bmin[0] + ((float)(x0 + i) + 0.5f) * s.vs[0];
+                            sum[i] = 0.0f;
+                            used[i] = 0;
+                        }
 #pragma unroll 1
-                            for (int t = 0; t < nsmp; ++t) {
-                                const float4 sm = bp.samples[t];
-                                const int ys = t / pd;
-                                const float2 pt = pterm[p * rowsets + ys];
-                                float u = m0 * (cx + sm.x) + pt.x;
-                                u = u + m8 * (cz + sm.z);
+                        for (int t = 0; t < nsmp; ++t) {
+                            const float4 sm = bp.samples[t];
+                            const int ys = t / pd;
+                            const float2 pt = pterm[p * rowsets + ys];
+                            const float zt = m8 * (cz + sm.z);
+                            const float lw = lam * sm.w;
+#pragma unroll
+                            for (int i = 0; i < G; ++i) {
+                                float u = m0 * (cx[i] + sm.x) + pt.x;
+                                u = u + zt;
                                 u = u + m12 * 1.0f;
                                 float L, contrib;
                                 if (kFast) {
@@ -900,14 +910,14 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
                                     const float4 c = recs[(p * rowsets + ys) * row_len + (int)bu + pad_l];
                                     const float Rv = fmaf(c.y, tx, c.x);
                                     L = fmaf(c.w, tx, c.z);
-                                    contrib = Rv * (lam * sm.w) * __builtin_amdgcn_rcpf(L);
+                                    contrib = Rv * lw * __builtin_amdgcn_rcpf(L);
                                 } else {
                                     // bp_voxel_sum's bilinear lookups and weight, operation for operation
                                     float bu;
                                     const float tx = cl_fract(u - 0.5f, &bu);
-                                    const int i = padded_index((int)bu, nu);
+                                    const int k = padded_index((int)bu, nu);
                                     const float4* pr = recs + (p * rowsets + ys) * 2 * row_len;
-                                    const float4 rr = pr[2 * i], ll = pr[2 * i + 1];
+                                    const float4 rr = pr[2 * k], ll = pr[2 * k + 1];
                                     const float L0 = lerp_pair(ll.x, ll.y, tx), L1 = lerp_pair(ll.z, ll.w, tx);
                                     L = L0 + (L1 - L0) * pt.y;
                                     const float R0 = lerp_pair(rr.x, rr.y, tx), R1 = lerp_pair(rr.z, rr.w, tx);
@@ -916,12 +926,16 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
                                     contrib = Rv * lam * wgt;
                                 }
                                 const bool ok = L > kEpsilon;
-                                sum += ok ? contrib : 0.0f;
-                                used += ok ? 1 : 0;
+                                sum[i] += ok ? contrib : 0.0f;
+                                used[i] += ok ? 1 : 0;
                             }
+                        }
+                        float out[G];
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
                             const int at = zz * width + xw + i;
                             float value = band[at];
-                            if (used != 0) value += (kMask ? band[at + kMaskOff] * sum : sum) / (float)used;
+                            if (used[i] != 0) value += (kMask ? band[at + kMaskOff] * sum[i] : sum[i]) / (float)used[i];
                             out[i] = S::round(value);
                             band[at] = out[i];
                         }
