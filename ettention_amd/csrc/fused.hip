@@ -97,6 +97,25 @@ int max_band(const MemberPlan& m, int planes, int threads)
 
 int max_band(const MemberPlan& m, int planes) { return max_band(m, planes, m.threads); }
 
+// The 80 KiB register back projection (dev::FusedLaunch::res == 4): LDS
+// floats of member m's band of `band` rows with `sets` record sets staged, and
+// the tallest band (at most the member's own) that fits; 0 when none does.
+size_t res2_lds_bytes(const MemberPlan& m, int band, int sets, bool mask)
+{
+    const int width = m.tile.planes * m.nx;
+    return sizeof(float) * (size_t)((mask ? dev::kRes2Band / 2 : 0) + ((band * width + 3) & ~3) +
+                                    m.tile.planes * m.rowsets * sets * (m.row_floats + 2));
+}
+int res2_band(const MemberPlan& m, int sets, bool mask)
+{
+    const int width = m.tile.planes * m.nx;
+    const int cap = mask ? dev::kRes2Band / 2 : dev::kRes2Band;
+    const int rec = m.tile.planes * m.rowsets * sets * (m.row_floats + 2);
+    const int room = dev::kRes2Lds - (mask ? dev::kRes2Band / 2 : 0) - rec - 3;
+    if (room < width) return 0;
+    return std::min(m.tile.band, std::min(cap / width, room / width));
+}
+
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
@@ -422,6 +441,8 @@ private:
         bool steady;
         int res;                // register back projection (dev::FusedLaunch::res)
         size_t lds_res;         // LDS bytes with every back projection's records staged
+        int band4;              // z rows per band that fit the 80 KiB register path (0: none)
+        size_t lds4;            // its LDS bytes
     };
     struct Schedule {
         std::vector<Launch> launches;
@@ -563,8 +584,9 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
     std::vector<const dev::FusedPass*> tables(length * members_.size(), nullptr);
     out.launches.clear();
     for (size_t k = 0; k < length; ++k) {
-        Launch l{work.size(), 0, 1, 0, true, 0, 0};
+        Launch l{work.size(), 0, 1, 0, true, 0, 0, 0, 0};
         bool fit_shape = true, fit_res = true;
+        int band4 = 1 << 30;
         int max_bp = 0;
         for (size_t i : order) {
             const MemberPlan& m = members_[i];
@@ -588,13 +610,23 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
             const bool same_x = threads_ % (m.tile.planes * m.nx / 4) == 0;
             fit_shape = fit_shape && same_x && res <= sizeof(float) * (size_t)dev::shape_lds_floats(threads_, m.vbytes);
             fit_res = fit_res && same_x && res <= sizeof(float) * (size_t)dev::kResLds;
+            band4 = same_x ? std::min(band4, res2_band(m, std::max(p.nbp, 1), mask_)) : 0;
+        }
+        if (band4 >= 1 && band4 < (1 << 30)) {
+            l.band4 = band4;
+            for (size_t i : order) {
+                const MemberPlan& m = members_[i];
+                if (k < (size_t)m.offset || k >= (size_t)m.offset + m.passes.size()) continue;
+                const dev::FusedPass& p = m.passes[k - m.offset];
+                l.lds4 = std::max(l.lds4, res2_lds_bytes(m, band4, std::max(p.nbp, 1), mask_));
+            }
         }
         // passes with one back projection keep the LDS path: in registers it
         // gained nothing with f32 storage (638 against 636 GUPS at 512^3) and
         // lost 13% with bf16 (the band's values and the next band's loads
         // exceed the VGPR budget of four waves per SIMD)
         if (reg_bp_ && !exact_ && !over_ && vec_ && !cap_ && !sr_ && max_bp > 1)
-            l.res = fit_shape ? 1 : threads_ == 512 && fit_res ? 2 : 0;
+            l.res = fit_shape ? 1 : threads_ == 512 && l.band4 > 0 ? 4 : threads_ == 512 && fit_res ? 2 : 0;
         l.groups = (int)(work.size() - l.work);
         out.launches.push_back(l);
     }
@@ -617,7 +649,9 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         const Launch& l = s.launches[k];
         if (l.groups == 0) continue;
         const dev::FusedLaunch fl{threads_, l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
-                                  tables + k * members_.size(), l.res ? l.lds_res : l.lds, sr_, l.res};
+                                  tables + k * members_.size(),
+                                  l.res == 4 ? l.lds4 : l.res ? l.lds_res : l.lds, sr_, l.res,
+                                  l.res == 4 ? l.band4 : 0};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
         dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));
@@ -1013,13 +1047,18 @@ void FusedBackProjectorImpl::backProject(const std::vector<uint32_t>& indices,
                                    m_.tile.planes, m_.tile.band, m_.nx, m_.row_floats, false, m_.threads,
                                    m_.rowsets * std::max(passes[k].nbp, 1), m_.vbytes);
         int res = 0;
+        const int band4 = m_.threads == 512 ? res2_band(m_, std::max(passes[k].nbp, 1), false) : 0;
         if (!m_.lds_bp && !m_.oversampled() && passes[k].nbp > 1 && m_.threads % (m_.tile.planes * m_.nx / 4) == 0)
             res = lds_res <= sizeof(float) * (size_t)dev::shape_lds_floats(m_.threads, m_.vbytes) ? 1
+                  : band4 > 0                                                                      ? 4
                   : m_.threads == 512 && lds_res <= sizeof(float) * (size_t)dev::kResLds           ? 2
                                                                                                      : 0;
+        const size_t used = res == 4 ? res2_lds_bytes(m_, band4, std::max(passes[k].nbp, 1), false)
+                            : res    ? lds_res
+                                     : lds;
         const dev::FusedLaunch fl{m_.threads, 1, (int)work.size(), dvol.as<dev::FusedVolume>(),
                                   dwork.as<dev::FusedWork>(), dtables.as<const dev::FusedPass*>() + k,
-                                  res ? lds_res : lds, false, res};
+                                  used, false, res, res == 4 ? band4 : 0};
         if (launch_ms) ETN_HIP(hipEventRecord(ev[2 * k], stream));
         dev::launch_fused(volume_->voxel_type(), true, false, false, false, m_.oversampled(), fl, stream);
         if (launch_ms) ETN_HIP(hipEventRecord(ev[2 * k + 1], stream));

@@ -66,19 +66,31 @@ struct FusedShape<256, kBytes> {
     static constexpr int kBand = 6144;
 };
 // The register back projection with every record set of a pass resident in
-// LDS (kernels with kRes == 2, 512 threads only): a 160 KiB array, so one
-// workgroup per CU (the 512-thread kernels with 4 or 8 rays per thread hold
-// only one anyway, by their registers), with the band of the 512-thread shape.
+// LDS (512 threads only): kRes == 2 takes a 160 KiB array, so one workgroup
+// per CU (the 512-thread kernels with 4 or 8 rays per thread hold only one
+// anyway, by their registers), with the band of the 512-thread shape;
+// kRes == 4 takes an 80 KiB array with bands of at most 4096 voxels (the host
+// passes a band height that fits) and 128 VGPRs, so that two workgroups share
+// a CU.
 constexpr int kResLds = 40960;
-template <int kThreads, int kBytes, bool kResident>
+constexpr int kRes2Lds = 20480;
+constexpr int kRes2Band = 4096;
+template <int kThreads, int kBytes, int kRes>
 struct FusedShapeOf {
     using type = FusedShape<kThreads, kBytes>;
 };
 template <int kBytes>
-struct FusedShapeOf<512, kBytes, true> {
+struct FusedShapeOf<512, kBytes, 2> {
     struct type {
         static constexpr int kLds = kResLds;
         static constexpr int kBand = FusedShape<512, kBytes>::kBand;
+    };
+};
+template <int kBytes>
+struct FusedShapeOf<512, kBytes, 4> {
+    struct type {
+        static constexpr int kLds = kRes2Lds;
+        static constexpr int kBand = kRes2Band;
     };
 };
 constexpr int kMaxThreads = 512;
@@ -176,8 +188,10 @@ struct FusedLaunch {
     // back projections applied in registers as the band is loaded (fast
     // kernels, nx a multiple of 4, no oversampling, stochastic rounding or
     // step limit), the record sets of all of a pass's back projections staged
-    // once per launch: 0 off, 1 in the shape's LDS array, 2 in the 160 KiB one
+    // once per launch: 0 off, 1 in the shape's LDS array, 2 in the 160 KiB one,
+    // 4 in the 80 KiB one with band_rows z rows per band
     int res;
+    int band_rows;               // > 0: z rows per band instead of each member's own
 };
 
 namespace {
@@ -448,14 +462,14 @@ __device__ __forceinline__ float bp_update_fast(const float4* pr, float xt, floa
 // roundings as on the LDS path, so the volumes are bit-identical.
 template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT, bool kSR,
           int kRes>
-__global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver && RPT == 4)) ? 4 : 1)
+__global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <= 2 || (kOver && RPT == 4)) ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
-                       const FusedPass* const* __restrict__ passes)
+                       const FusedPass* const* __restrict__ passes, int band_rows)
 {
     static_assert(kRes == 0 || (kVec && kFast && !kOver && !kSR), "register back projection: fast vector kernels");
-    static_assert(kRes != 2 || kThreads == 512, "resident record sets: 512-thread workgroups");
+    static_assert((kRes != 2 && kRes != 4) || kThreads == 512, "resident record sets: 512-thread workgroups");
     using IO = GroupIo<S, kVec>;
-    using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes == 2>::type;
+    using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes>::type;
     constexpr int G = IO::G;
     constexpr int kMaskOff = Shape::kBand / 2;          // LDS floats from a voxel to its mask weight
     constexpr int kBandCap = kMask ? kMaskOff : Shape::kBand;
@@ -465,6 +479,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
     __shared__ __attribute__((aligned(16))) float lds[Shape::kLds];
     const FusedWork item = work[blockIdx.x];
     const ETN_GLOBAL FusedVolume& fv = global_ptr(vols)[item.member];
+    const int fband = band_rows > 0 ? band_rows : fv.band;   // z rows per band
     const ETN_GLOBAL FusedPass* pass = global_ptr(global_ptr(passes)[item.member]);
     const int j = item.plane;                           // planes j .. j + np - 1
     const int tid = threadIdx.x;
@@ -476,7 +491,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
     // band, then per plane and row set (m4 py, ty) of the current back projection
     const int rowsets = kOver ? fv.rowsets : 1;
     float* mband = lds + kMaskOff;                      // kMask: the band's mask weights
-    float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskOff : 0) + ((fv.band * width + 3) & ~3));
+    float4* recs = reinterpret_cast<float4*>(lds + (kMask ? kMaskOff : 0) + ((fband * width + 3) & ~3));
     constexpr int kRecStride = kFast ? 1 : 2;   // float4 per column record
     ETN_GLOBAL typename S::T* vol = global_ptr(static_cast<typename S::T*>(fv.data));
     const ETN_GLOBAL uint8_t* mask = kMask ? global_ptr(fv.mask) : nullptr;
@@ -564,7 +579,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
         }
     }
 
-    const int nbands = (nz + fv.band - 1) / fv.band;
+    const int nbands = (nz + fband - 1) / fband;
 
     // v coordinate of plane j + p under back projection bp for a sample with y
     // offset dy, constant over the plane (checked on the host): returns
@@ -627,8 +642,8 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (RPT <= 2 || (kOver 
 
     auto band_range = [&](int bi, int& z0, int& z1) {
         const int b = pass->zdir >= 0 ? bi : nbands - 1 - bi;
-        z0 = b * fv.band;
-        z1 = min(z0 + fv.band, nz);
+        z0 = b * fband;
+        z1 = min(z0 + fband, nz);
     };
 
     // the vector path keeps the next band in registers while this band is
@@ -1128,25 +1143,25 @@ template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, 
           int kRes = 0>
 void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
 {
-    using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes == 2>::type;
+    using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes>::type;
     if (l.lds > sizeof(float) * (size_t)Shape::kLds) fail("fused plane kernel: tile exceeds the LDS array");
     const dim3 grid(l.groups), block(kThreads);
     switch (l.rpt) {
         case 1:
             fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
             break;
         case 2:
             fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
             break;
         case 4:
             fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
             break;
         default:
             fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes);
+                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
             break;
     }
 }
@@ -1176,6 +1191,10 @@ void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t st
             if (vec && !cap && !l.sr) {
                 if (l.res == 2 && l.threads == 512) {
                     launch_fused_shape<S, true, false, true, kMask, false, 512, false, 2>(l, stream);
+                    return;
+                }
+                if (l.res == 4 && l.threads == 512 && l.band_rows > 0) {
+                    launch_fused_shape<S, true, false, true, kMask, false, 512, false, 4>(l, stream);
                     return;
                 }
                 if (l.res == 1 && l.threads == 512) {

@@ -63,6 +63,11 @@ CASES = {
     "sirt_loc": ((64, 32, 48), 21, dict(algorithm="sirt", use_long_object_compensation=True)),
     "block8_bf16": ((64, 64, 32), 41, dict(algorithm="blockiterative", block_size=8, voxel_type="bfloat16")),
     "sart_half": ((64, 48, 40), 31, dict(voxel_type="half")),
+    # 512 wide: 8 record sets of 512-voxel rows no longer fit the workgroup
+    # shape's LDS, so one-plane tiles take the 80 KiB register path (bands of
+    # at most 7 rows) and wider tiles the 160 KiB one
+    "block8_wide": ((512, 8, 24), 61, dict(algorithm="blockiterative", block_size=8)),
+    "block8_wide_bf16": ((512, 8, 24), 41, dict(algorithm="blockiterative", block_size=8, voxel_type="bfloat16")),
 }
 
 
@@ -78,7 +83,7 @@ def test_register_back_projection_equals_lds_back_projection(name, threads):
     assert reg_norm == lds_norm
 
 
-@pytest.mark.parametrize("name", ["sart", "block8"])
+@pytest.mark.parametrize("name", ["sart", "block8", "block8_wide"])
 def test_register_back_projection_with_a_mask(name):
     shape, ntilts, kw = CASES[name]
     nx, ny, nz = shape
@@ -100,6 +105,20 @@ def test_register_back_projection_every_tile(tile):
     print("register bp tile", tile, flush=True)
     ref, _ = run(case, flags=LDS | E.ETN_RECON_FLAG_NO_AUTOTUNE)
     got, _ = run(case, flags=E.ETN_RECON_FLAG_NO_AUTOTUNE, tile=tile)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("tile", [(1, 1), (1, 3), (1, 12), (2, 2), (4, 1)])
+def test_register_back_projection_wide_tiles(tile):
+    # 512 wide, 8 back projections per pass: one plane per tile takes the
+    # 80 KiB register path with the band cut to what fits beside the records,
+    # two planes the 160 KiB one, four (whose records exceed even that) the
+    # LDS path
+    shape, ntilts, kw = CASES["block8_wide"]
+    case = inputs(shape, ntilts, 7500, **kw)
+    print("register bp wide tile", tile, flush=True)
+    ref, _ = run(case, flags=LDS | E.ETN_RECON_FLAG_NO_AUTOTUNE, threads=512)
+    got, _ = run(case, flags=E.ETN_RECON_FLAG_NO_AUTOTUNE, threads=512, tile=tile)
     np.testing.assert_array_equal(got, ref)
 
 
