@@ -531,54 +531,6 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
         return w < row_hi[j + p] && sub < fp_n;
     };
 
-    // ---- ray set-up: dda_begin exactly as the generic FP kernel ----
-    PlaneRay ray[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int id = k * kThreads + tid;
-        ray[k].live = 0;
-        ray[k].acc = 0.0f;
-        ray[k].accl = 0.0f;
-        ray[k].off = 0;
-        int q, p, w, u, sub;
-        if (id < nrays && decode(id, q, p, w, u, sub)) {
-            const ETN_GLOBAL FusedFp& f = pass->fp[q];
-            RayParams g;
-            for (int a = 0; a < 3; ++a) {
-                g.source[a] = f.src[a];
-                g.h[a] = f.h[a];
-                g.v[a] = f.v[a];
-                g.ray[a] = f.ray[a];
-            }
-            const DdaConst c = dda_const(g.ray, s.vs);
-            float src[3];
-            if (kOver) {
-                // the generic FP's sub-ray loop: sd = 1 / n, offset sd / 2; sub-ray (si, 0)
-                const float sd = 1.0f / (float)fp_n;
-                fp_ray_source(g, (uint32_t)u, (uint32_t)w, (uint32_t)sub, 0, sd, sd / 2.0f, src);
-            } else {
-                fp_ray_source(g, (uint32_t)u, (uint32_t)w, 0, 0, 1.0f, 0.5f, src);
-            }
-            DdaRay r;
-            dda_begin(s, c, src, r);
-            if (r.live && r.vox[1] != j + p) {
-                atomicAdd(fv.diag, 1);
-                r.live = 0;
-            }
-            ray[k].vx = r.vox[0];
-            ray[k].vz = r.vox[2];
-            ray[k].rx = r.res[0];
-            ray[k].rz = r.res[2];
-            ray[k].brk = r.brk;
-            ray[k].live = r.live;
-            ray[k].stepx = c.step[0];
-            ray[k].stepz = c.step[2];
-            ray[k].sx = c.sign[0];
-            ray[k].sz = c.sign[2];
-            ray[k].off = p * nx;
-        }
-    }
-
     const int nbands = (nz + fband - 1) / fband;
 
     // v coordinate of plane j + p under back projection bp for a sample with y
@@ -668,8 +620,74 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
         }
     };
 
+    // with at most two rays per thread the first band's loads go out before
+    // the rays are set up (more rays: their set-up would hold the loads'
+    // registers too)
     int z0, z1;
     band_range(0, z0, z1);
+    if (RPT <= 2) prefetch(z0, z1, 0, kPre);
+
+    // the measured projection's value in its storage type (a uniform branch)
+    auto measured_at = [](const ETN_GLOBAL FusedFp& f, size_t i) -> float {
+        if (f.measured_type == ETN_VOXEL_FLOAT16) return F16::load(global_ptr(static_cast<const __half*>(f.measured)), i);
+        if (f.measured_type == ETN_VOXEL_BFLOAT16)
+            return BF16::load(global_ptr(static_cast<const uint16_t*>(f.measured)), i);
+        return global_ptr(static_cast<const float*>(f.measured))[i];
+    };
+
+    // ---- ray set-up: dda_begin exactly as the generic FP kernel ----
+    PlaneRay ray[RPT];
+    // with at most two rays per thread each ray's measured pixel is loaded
+    // here, so that the residual does not wait for it after the last band
+    constexpr bool kEarly = RPT <= 2;
+    [[maybe_unused]] float meas[kEarly ? RPT : 1];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int id = k * kThreads + tid;
+        ray[k].live = 0;
+        ray[k].acc = 0.0f;
+        ray[k].accl = 0.0f;
+        ray[k].off = 0;
+        int q, p, w, u, sub;
+        if (id < nrays && decode(id, q, p, w, u, sub)) {
+            const ETN_GLOBAL FusedFp& f = pass->fp[q];
+            if (kEarly && sub == 0 && !f.raw) meas[k] = measured_at(f, (size_t)w * nu + u);
+            RayParams g;
+            for (int a = 0; a < 3; ++a) {
+                g.source[a] = f.src[a];
+                g.h[a] = f.h[a];
+                g.v[a] = f.v[a];
+                g.ray[a] = f.ray[a];
+            }
+            const DdaConst c = dda_const(g.ray, s.vs);
+            float src[3];
+            if (kOver) {
+                // the generic FP's sub-ray loop: sd = 1 / n, offset sd / 2; sub-ray (si, 0)
+                const float sd = 1.0f / (float)fp_n;
+                fp_ray_source(g, (uint32_t)u, (uint32_t)w, (uint32_t)sub, 0, sd, sd / 2.0f, src);
+            } else {
+                fp_ray_source(g, (uint32_t)u, (uint32_t)w, 0, 0, 1.0f, 0.5f, src);
+            }
+            DdaRay r;
+            dda_begin(s, c, src, r);
+            if (r.live && r.vox[1] != j + p) {
+                atomicAdd(fv.diag, 1);
+                r.live = 0;
+            }
+            ray[k].vx = r.vox[0];
+            ray[k].vz = r.vox[2];
+            ray[k].rx = r.res[0];
+            ray[k].rz = r.res[2];
+            ray[k].brk = r.brk;
+            ray[k].live = r.live;
+            ray[k].stepx = c.step[0];
+            ray[k].stepz = c.step[2];
+            ray[k].sx = c.sign[0];
+            ray[k].sz = c.sign[2];
+            ray[k].off = p * nx;
+        }
+    }
+
     if (!kCap) {
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
@@ -680,7 +698,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
             r.dzb = 4 * r.sz * width;
         }
     }
-    prefetch(z0, z1, 0, kPre);
+    if (RPT > 2) prefetch(z0, z1, 0, kPre);
     if constexpr (kRes != 0) {
         // every back projection's records, before the first band needs them
         if (nbp > 0) {
@@ -1065,14 +1083,6 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
         __syncthreads();
     }
 
-    // the measured projection's value in its storage type (a uniform branch)
-    auto measured_at = [](const ETN_GLOBAL FusedFp& f, size_t i) -> float {
-        if (f.measured_type == ETN_VOXEL_FLOAT16) return F16::load(global_ptr(static_cast<const __half*>(f.measured)), i);
-        if (f.measured_type == ETN_VOXEL_BFLOAT16)
-            return BF16::load(global_ptr(static_cast<const uint16_t*>(f.measured)), i);
-        return global_ptr(static_cast<const float*>(f.measured))[i];
-    };
-
     // ---- residual (+ long-object compensation) per pixel ----
     // oversampled: every traced sub-ray's sums go through LDS (free after the
     // last band); the pixel's first slot adds sub-ray (si, sj)'s sums, those of
@@ -1113,7 +1123,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
                 l = f.loc_c;
             }
             const size_t i = (size_t)w * nu + u;
-            global_ptr(f.residual)[i] = f.raw ? pv : measured_at(f, i) - pv;
+            global_ptr(f.residual)[i] = f.raw ? pv : (kEarly ? meas[k] : measured_at(f, i)) - pv;
             global_ptr(f.length)[i] = l;
         }
     }
