@@ -192,6 +192,9 @@ struct FusedLaunch {
     // 4 in the 80 KiB one with band_rows z rows per band
     int res;
     int band_rows;               // > 0: z rows per band instead of each member's own
+    // non-null: nothing is launched; the number of workgroups of the kernel
+    // this launch selects that the device holds at once is stored here
+    int* resident;
 };
 
 namespace {
@@ -1155,25 +1158,23 @@ void launch_fused_shape(const FusedLaunch& l, hipStream_t stream)
 {
     using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes>::type;
     if (l.lds > sizeof(float) * (size_t)Shape::kLds) fail("fused plane kernel: tile exceeds the LDS array");
-    const dim3 grid(l.groups), block(kThreads);
+    using Kernel = void (*)(const FusedVolume*, const FusedWork*, const FusedPass* const*, int);
+    Kernel kernel;
     switch (l.rpt) {
-        case 1:
-            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
-            break;
-        case 2:
-            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
-            break;
-        case 4:
-            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
-            break;
-        default:
-            fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8, kSR, kRes>
-                <<<grid, block, 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
-            break;
+        case 1: kernel = fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 1, kSR, kRes>; break;
+        case 2: kernel = fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 2, kSR, kRes>; break;
+        case 4: kernel = fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 4, kSR, kRes>; break;
+        default: kernel = fused_plane_kernel<S, kVec, kCap, kFast, kMask, kOver, kThreads, 8, kSR, kRes>; break;
     }
+    if (l.resident) {
+        int per_cu = 0, dev = 0, cus = 0;
+        ETN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0));
+        ETN_HIP(hipGetDevice(&dev));
+        ETN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        *l.resident = per_cu * cus;
+        return;
+    }
+    kernel<<<dim3(l.groups), dim3(kThreads), 0, stream>>>(l.vols, l.work, l.passes, l.band_rows);
 }
 
 // The oversampled kernels exist for 512-thread workgroups only.
