@@ -105,6 +105,39 @@ def test_fused_equals_generic_on_mrc_geometry(shape, det):
     assert rel_l2(fast, ref) <= 1e-5
 
 
+def test_rounds_of_plane_groups(synthetic):
+    # 1024 one-plane groups per launch, more than the device holds at once:
+    # the rounds are dispatched interleaved, which must not change a voxel
+    nx, ny, nz = 512, 1024, 8
+    truth = S.ellipsoid_phantom((nz, ny, nx), seed=78)
+    vg = E.VolumeGeometry.make((nx, ny, nz))
+    base = E.ScannerGeometry.mrc_base(nx, ny)
+    tilts = S.tilt_series(60.0, 15.0)
+    truth_t = dev(truth)
+    projs = torch.zeros((len(tilts), ny, nx), dtype=torch.float32, device="cuda")
+    scratch = torch.zeros((ny, nx), dtype=torch.float32, device="cuda")
+    for i, t in enumerate(tilts):
+        A.forward_project(vg, truth_t, A.rotate(base, float(t)), projs[i], scratch)
+    vols = {}
+    for name, flags, tile in (("exact", E.ETN_RECON_FLAG_EXACT_ARITHMETIC, (1, 8)),
+                              ("generic", E.ETN_RECON_FLAG_GENERIC_KERNELS, None),
+                              ("fast", 0, (1, 8)), ("fast_two_planes", 0, (2, 4))):
+        r = A.BlockIterativeReconstructionOperator(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs,
+                                                   iterations=2, algorithm="sart", lam=0.3,
+                                                   projection_iteration="random",
+                                                   flags=flags | E.ETN_RECON_FLAG_NO_AUTOTUNE)
+        if tile is not None:
+            r.set_workgroup_threads(512)
+            r.set_tile(*tile)
+            assert r.tile() == tile
+        r.run()
+        r.synchronize()
+        vols[name] = r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy()
+    np.testing.assert_array_equal(vols["exact"], vols["generic"])
+    np.testing.assert_array_equal(vols["fast"], vols["fast_two_planes"])
+    assert rel_l2(vols["fast"], vols["generic"]) <= 1e-5
+
+
 def test_ineligible_configurations_use_generic_kernels(synthetic):
     gt, tilts, projs = synthetic
     sub = slice(None, None, 20)
