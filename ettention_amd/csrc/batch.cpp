@@ -50,7 +50,8 @@ BatchReconstruction::BatchReconstruction(const etn_reconstruction_desc* members,
         for (uint32_t i = 0; i < count; ++i)
             if ((members[i].flags & ETN_RECON_FLAG_STOCHASTIC_ROUNDING) && slots_[i].engine < 0)
                 fail("batch member " + std::to_string(i) + ": stochastic rounding needs the fast fused kernels "
-                     "(f16 or bf16 storage, no mask, no oversampling, nx a multiple of 4, nx + nz <= 2049)");
+                     "(f16 or bf16 storage, no mask, no oversampling, nx a multiple of 4, rays that cannot "
+                     "reach the 2048-step limit)");
     } catch (...) {
         ops_.clear();
         engines_.clear();

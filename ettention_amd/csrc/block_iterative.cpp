@@ -105,7 +105,7 @@ BlockIterativeReconstructionOperator::BlockIterativeReconstructionOperator(const
     }
     if ((d.flags & ETN_RECON_FLAG_STOCHASTIC_ROUNDING) && !fused_)
         fail("stochastic rounding needs the fast fused kernels: f16 or bf16 storage, no mask, no oversampling, "
-             "single-axis tilt, nx a multiple of 4 and nx + nz <= 2049");
+             "single-axis tilt, nx a multiple of 4 and rays that cannot reach the 2048-step limit");
     tuned_ = (d.flags & ETN_RECON_FLAG_NO_AUTOTUNE) != 0;
 }
 

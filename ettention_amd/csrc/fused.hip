@@ -15,7 +15,8 @@
 namespace etn {
 namespace dev {
 
-// vec: every member's nx is a multiple of 4; cap: some member has nx + nz > 2049;
+// vec: every member's nx is a multiple of 4; cap: some member's rays can reach
+// the 2048-step limit (step_limit_can_bind);
 // exact: the generic kernels' operations and rounding order; mask: the launch
 // carries prior-knowledge masks; over: oversampled FP and BP
 void launch_fused(etn_voxel_type vt, bool vec, bool cap, bool exact, bool mask, bool over, const FusedLaunch& l,
@@ -119,8 +120,8 @@ int res2_band(const MemberPlan& m, int sets, bool mask)
 // Dispatch order of a launch's workgroups.  The plane groups of a launch that
 // the device cannot hold at once run in rounds; in plane order the first round
 // then streams only the first part of every z row of the volume (a contiguous
-// nx * ny * voxel-size slice per row), and with 4 MiB slices (1024 x 1024 f32,
-// 2048 x 512 f32) that part alone left HBM half idle: 1024 x 1024 x 256 f32
+// nx * ny * voxel-size slice per row), and with 4 MiB slices (1024 x 1024
+// f32) that part alone left HBM half idle: 1024 x 1024 x 256 f32
 // SART ran at 485 GUPS against 620 with this order.  Here the rounds' groups
 // are interleaved: with r rounds of n groups each, dispatch i runs group
 // (i mod r) * n + i / r, so every round spans the whole slice in r contiguous
@@ -213,6 +214,40 @@ bool fast_row_span(const VolumeProperties& vp, const std::vector<const ParallelS
     pad_l = std::max(dev::kRowPad, -bu_lo);
     row_len = pad_l + std::max(nu + dev::kRowPad, bu_hi + 1);
     return true;
+}
+
+// Whether the reference's 2048-step limit (2048 voxels per ray and volume
+// sweep) can bind for one reconstruction.  A ray confined to a y plane visits
+// one voxel more than it crosses x and z voxel borders.  With a chord spanning
+// dx voxels in x and dz in z (dx <= nx, dz <= nz, dx / dz fixed by the ray
+// direction in voxel units) it crosses at most floor(dx) + 1 x borders and
+// floor(dz) + 1 z borders, and never more than nx + nz - 2 in all.  Only
+// when some projection's bound exceeds 2040 voxels (8 below the limit, for
+// the traversal's float rounding) do the launches take the step-limited
+// kernels; otherwise the limit is never reached and the kernels without it
+// compute the same sums.  (2048-wide volumes at tilts up to 60 degrees,
+// for example, stay far below it.)
+bool step_limit_can_bind(BlockIterativeReconstructionOperator& op)
+{
+    const VolumeProperties vp = whole_volume(op.getReconstructedVolume()->properties());
+    const double nx = (double)vp.nx(), nz = (double)vp.nz();
+    if (nx + nz - 1.0 <= 2048.0) return false;
+    const Vec3f vs = vp.voxel_size();
+    for (uint32_t b = 0; b < op.GetBlockCount(); ++b)
+        for (uint32_t p = 0; p < op.getBlockSize(b); ++p) {
+            const Vec3f r = op.geometry(op.getPhysicalProjectionIndex(b, p)).ray_direction();
+            const double ax = std::fabs((double)r.x) / (double)vs.x, az = std::fabs((double)r.z) / (double)vs.z;
+            double dx = nx, dz = nz;
+            if (az == 0.0) dz = 0.0;
+            else if (ax == 0.0) dx = 0.0;
+            else {
+                dx = std::min(nx, nz * ax / az);
+                dz = std::min(nz, nx * az / ax);
+            }
+            const double visits = std::min(nx + nz - 1.0, std::floor(dx) + std::floor(dz) + 3.0);
+            if (visits > 2040.0) return true;
+        }
+    return false;
 }
 
 // Eligibility of one reconstruction and its plane/row tables; false when it
@@ -1188,7 +1223,8 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
             continue;
         MemberPlan m;
         if (!plan_member(*members[i].op, d, exact, m)) continue;
-        if (d_sr && m.nx + m.nz > 2049) continue;
+        const bool m_cap = step_limit_can_bind(*members[i].op);
+        if (d_sr && m_cap) continue;
         m.sr = d_sr;
         m.lds_bp = d_lds;
         if (plans.empty()) {
@@ -1198,7 +1234,7 @@ std::unique_ptr<FusedPlaneEngine> FusedPlaneEngine::create(const std::vector<Mem
             over = m.oversampled();
             sr = m.sr;
         }
-        cap = cap || m.nx + m.nz > 2049;
+        cap = cap || m_cap;
         plans.push_back(std::move(m));
         (*taken)[i] = true;
     }

@@ -222,8 +222,9 @@ struct PlaneRay {
 // Advances a ray while it is inside the volume and inside band [zlo, zhi).  A
 // ray leaving the volume is finished; one leaving the band resumes in the next
 // band with unchanged state.  kCap keeps the reference's 2048-step limit; it is
-// dropped when nx + nz <= 2049, since every step moves vx or vz monotonically by
-// one, so a ray can take at most nx + nz - 1 steps inside the volume.
+// dropped when no ray of the launch can visit more than 2040 voxels (the host
+// bounds that by the rays' directions and nx + nz - 1, since every step moves
+// vx or vz monotonically by one).
 // kFast accumulates dt * value with a fused multiply-add (one rounding instead
 // of two, as an OpenCL compiler may contract it).  kMask weights the ray
 // length by the voxel's mask weight, stored kMaskOff floats after the voxel.
@@ -1232,7 +1233,7 @@ void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t st
             }
         }
         fail("stochastic rounding needs 16-bit storage, fast arithmetic, no mask, no oversampling, nx a multiple "
-             "of 4 and nx + nz <= 2049");
+             "of 4 and rays that cannot reach the 2048-step limit");
     }
     if (vec && !cap) launch_fused_rpt<S, true, false, kFast, kMask, kOver>(l, stream);
     else if (vec) launch_fused_rpt<S, true, true, kFast, kMask, kOver>(l, stream);

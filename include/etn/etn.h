@@ -192,7 +192,9 @@ enum {
                                                    projection and the value, so runs are reproducible) instead
                                                    of to nearest even.  Extension (the reference rounds to
                                                    nearest); needs the fast fused kernels: f16 or bf16 storage,
-                                                   no mask, no oversampling, nx a multiple of 4, nx + nz <= 2049;
+                                                   no mask, no oversampling, nx a multiple of 4, rays that
+                                                   cannot reach the 2048-step limit (nx + nz <= 2049 always
+                                                   qualifies);
                                                    creation fails otherwise */,
     ETN_RECON_FLAG_16BIT_PROJECTIONS = 1u << 5,  /* the measured projections are stored in the volume's 16-bit
                                                    type (voxel_type f16 or bf16): `projections` addresses

@@ -2,10 +2,13 @@
 
 * The DDA step limit (DDARaytracer.hl:85-108, at most 2048 steps per ray): a
   thin 1152 x 2 x 1152 slab whose oblique rays cross more than 2048 voxel
-  borders.  nx + nz > 2049, so the fused kernels run their step-limited
-  variant.  Generic forward projections against the oracle within 1e-6; SART
+  borders, so the fused kernels run their step-limited variant.  Generic forward projections against the oracle within 1e-6; SART
   through the generic kernels against the oracle within 1e-5; exact fused
   arithmetic bit-identical to the generic kernels; fast arithmetic within 1e-5.
+* A 1536 x 2 x 600 volume at tilts up to 60 degrees: nx + nz > 2049, but no
+  ray can reach the limit, so the fused kernels run without it; exact fused
+  arithmetic bit-identical to the generic kernels (which keep it), fast
+  within 1e-5, the generic kernels within 1e-5 of the oracle.
 * x-axis tilt (SingleAxisTiltRotator.cpp:62-78): rays leave the y planes, so
   the generic kernels run; SART, SIRT and weighted back projection against the
   oracle within 1e-5."""
@@ -80,6 +83,36 @@ def test_step_limit_reconstruction(cap_case):
         vols[name] = r.get_reconstructed_volume().cpu().numpy()
     spec = O.ReconSpec(O.Volume.make(CAP_SHAPE), O.mrc_base_scanner(nx, ny), CAP_TILTS, projs, block_size=1,
                        order="random", lam=0.3)
+    ref = spec.run(np.zeros((nz, ny, nx), np.float32), 2)
+    assert rel_l2(vols["generic"], ref) <= 1e-5
+    assert np.array_equal(vols["exact"], vols["generic"])
+    assert rel_l2(vols["fast"], vols["generic"]) <= 1e-5
+
+
+def test_wide_volume_below_the_step_limit():
+    # nx + nz > 2049, but at tilts up to 60 degrees a ray visits at most
+    # floor(600 * tan(60)) + 600 + 3 = 1642 voxels: the fused kernels run
+    # without the step limit and must still match the generic kernels, which
+    # keep it
+    shape = (1536, 2, 600)
+    nx, ny, nz = shape
+    tilts = np.array([-60, -35, -10, 0, 15, 30, 50, 60], np.float32)
+    rng = np.random.default_rng(22)
+    truth = rng.uniform(0.0, 1.0, (nz, ny, nx)).astype(np.float32)
+    ov = O.Volume.make(shape)
+    obase = O.mrc_base_scanner(nx, ny)
+    projs = np.stack([O.forward_project(ov, truth, O.rotate(obase, float(t)))[0] for t in tilts])
+    print("wide volume SART", shape, flush=True)
+    kw = dict(volume=E.VolumeGeometry.make(shape), base_geometry=E.ScannerGeometry.mrc_base(nx, ny),
+              tilt_angles=tilts, projections=dev(projs), algorithm="sart", lam=0.3, projection_iteration="random")
+    vols = {}
+    for name, flags in (("generic", E.ETN_RECON_FLAG_GENERIC_KERNELS), ("exact", E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
+                        ("fast", 0)):
+        r = A.BlockIterativeReconstructionOperator(flags=flags, **kw)
+        assert r.uses_fused_path() == (name != "generic"), name
+        r.run_iterations(2)
+        vols[name] = r.get_reconstructed_volume().cpu().numpy()
+    spec = O.ReconSpec(ov, obase, tilts, projs, block_size=1, order="random", lam=0.3)
     ref = spec.run(np.zeros((nz, ny, nx), np.float32), 2)
     assert rel_l2(vols["generic"], ref) <= 1e-5
     assert np.array_equal(vols["exact"], vols["generic"])
