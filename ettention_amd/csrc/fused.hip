@@ -142,6 +142,24 @@ void interleave_rounds(dev::FusedWork* work, int groups, int resident)
     std::copy(out.begin(), out.end(), work);
 }
 
+// The 160 KiB LDS back projection (dev::FusedLaunch::res == 3): the tallest
+// band of member m (one record set per plane and row set staged) within the
+// 512-thread shape's band cap, and its LDS bytes.
+int res3_band(const MemberPlan& m, bool mask)
+{
+    const int width = m.tile.planes * m.nx;
+    const int masked = mask ? dev::shape_band_floats(dev::kMaxThreads, true, m.vbytes) : 0;
+    const int rec = m.tile.planes * m.rowsets * (m.row_floats + 2);
+    const int room = dev::kResLds - masked - rec - 3;
+    if (room < width) return 0;
+    return std::min(m.nz, std::min(dev::shape_band_floats(dev::kMaxThreads, mask, m.vbytes) / width, room / width));
+}
+size_t res3_lds_bytes(const MemberPlan& m, int band, bool mask)
+{
+    return sizeof(float) * (size_t)dev::fused_lds_floats(m.tile.planes, band, m.nx, m.row_floats, mask,
+                                                          dev::kMaxThreads, m.rowsets, m.vbytes);
+}
+
 bool tile_fits(const MemberPlan& m, const Tile& t)
 {
     return t.band >= 1 && t.band <= max_band(m, t.planes) &&
@@ -501,8 +519,8 @@ private:
         bool steady;
         int res;                // register back projection (dev::FusedLaunch::res)
         size_t lds_res;         // LDS bytes with every back projection's records staged
-        int band4;              // z rows per band that fit the 80 KiB register path (0: none)
-        size_t lds4;            // its LDS bytes
+        int band4;              // z rows per band of the 80 KiB register path (res 4) or the
+        size_t lds4;            // 160 KiB LDS path (res 3), 0: none; its LDS bytes
     };
     struct Schedule {
         std::vector<Launch> launches;
@@ -527,6 +545,7 @@ private:
     bool sr_;                           // stochastic rounding to 16-bit storage
     bool reg_bp_;                       // register back projection where it applies
     int threads_ = dev::kMaxThreads;    // workgroup shape of every launch
+    int dev_cus_ = 0;                   // compute units of the device
     DeviceBuffer diag_, vols_, passes_;
     std::vector<size_t> pass_base_;     // each member's first pass in passes_
     Schedule sched_;
@@ -539,6 +558,9 @@ FusedEngineImpl::FusedEngineImpl(std::vector<MemberPlan> members, etn_voxel_type
 {
     diag_.allocate(sizeof(int));
     ETN_HIP(hipMemsetAsync(diag_.get(), 0, sizeof(int), stream));
+    int dev = 0;
+    ETN_HIP(hipGetDevice(&dev));
+    ETN_HIP(hipDeviceGetAttribute(&dev_cus_, hipDeviceAttributeMultiprocessorCount, dev));
     upload(stream);
 }
 
@@ -691,19 +713,44 @@ void FusedEngineImpl::build_schedule(const std::vector<size_t>& ids, Schedule& o
         if (reg_bp_ && !exact_ && !over_ && vec_ && !cap_ && !sr_ && max_bp > 1)
             l.res = fit_shape ? 1 : threads_ == 512 && l.band4 > 0 ? 4 : threads_ == 512 && fit_res ? 2 : 0;
         l.groups = (int)(work.size() - l.work);
-        if (l.groups > 0) {
-            const long key = ((long)l.rpt << 32) | ((long)l.res << 16) | (long)l.band4;
+        auto resident_of = [&](const Launch& c) {
+            const long key = ((long)c.rpt << 32) | ((long)c.res << 16) | (long)c.band4;
             auto it = resident.find(key);
             if (it == resident.end()) {
                 int r = 0;
-                const dev::FusedLaunch q{threads_, l.rpt, l.groups, nullptr, nullptr, nullptr,
-                                         l.res == 4 ? l.lds4 : l.res ? l.lds_res : l.lds, sr_, l.res,
-                                         l.res == 4 ? l.band4 : 0, &r};
+                const dev::FusedLaunch q{threads_, c.rpt, c.groups, nullptr, nullptr, nullptr,
+                                         c.res == 4 || c.res == 3 ? c.lds4 : c.res ? c.lds_res : c.lds, sr_, c.res,
+                                         c.res == 4 || c.res == 3 ? c.band4 : 0, &r};
                 dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, q, stream);
                 it = resident.emplace(key, r).first;
             }
+            return it->second;
+        };
+        // a launch on the LDS path whose registers hold one 512-thread
+        // workgroup per CU takes the 160 KiB array where that gives taller
+        // bands than the tiles (wide volumes, whose records fill most of the
+        // 64 KiB shape)
+        if (l.groups > 0 && l.res == 0 && threads_ == 512 && vec_ && !cap_ && !over_ && !sr_ &&
+            resident_of(l) <= dev_cus_) {
+            int band3 = 1 << 30, tile_band = 1 << 30;
+            size_t lds3 = 0;
+            for (const auto& sg : segments) {
+                const MemberPlan& m = members_[work[sg.first].member];
+                band3 = std::min(band3, res3_band(m, mask_));
+                tile_band = std::min(tile_band, m.tile.band);
+            }
+            for (const auto& sg : segments)
+                lds3 = std::max(lds3, res3_lds_bytes(members_[work[sg.first].member], band3, mask_));
+            if (band3 > tile_band && band3 < (1 << 30)) {
+                l.res = 3;
+                l.band4 = band3;
+                l.lds4 = lds3;
+            }
+        }
+        if (l.groups > 0) {
+            const int held = resident_of(l);
             // each member's groups on their own (a batch keeps its largest-first order)
-            for (const auto& sg : segments) interleave_rounds(work.data() + sg.first, (int)sg.second, it->second);
+            for (const auto& sg : segments) interleave_rounds(work.data() + sg.first, (int)sg.second, held);
         }
         out.launches.push_back(l);
     }
@@ -727,8 +774,8 @@ void FusedEngineImpl::enqueue(const Schedule& s, size_t first, size_t last, hipS
         if (l.groups == 0) continue;
         const dev::FusedLaunch fl{threads_, l.rpt, l.groups, vols_.as<dev::FusedVolume>(), work + l.work,
                                   tables + k * members_.size(),
-                                  l.res == 4 ? l.lds4 : l.res ? l.lds_res : l.lds, sr_, l.res,
-                                  l.res == 4 ? l.band4 : 0};
+                                  l.res == 4 || l.res == 3 ? l.lds4 : l.res ? l.lds_res : l.lds, sr_, l.res,
+                                  l.res == 4 || l.res == 3 ? l.band4 : 0};
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k], stream));
         dev::launch_fused(vt_, vec_, cap_, exact_, mask_, over_, fl, stream);
         if (ev) ETN_HIP(hipEventRecord((*ev)[2 * k + 1], stream));

@@ -71,7 +71,11 @@ struct FusedShape<256, kBytes> {
 // anyway, by their registers), with the band of the 512-thread shape;
 // kRes == 4 takes an 80 KiB array with bands of at most 4096 voxels (the host
 // passes a band height that fits) and 128 VGPRs, so that two workgroups share
-// a CU.
+// a CU.  kRes == 3 keeps the back projection on the band in LDS but takes the
+// 160 KiB array too, for launches whose registers hold one 512-thread
+// workgroup per CU anyway (4 or 8 rays per thread): wide volumes, whose
+// detector records fill most of the 64 KiB shape, then get taller bands (the
+// host passes the height).
 constexpr int kResLds = 40960;
 constexpr int kRes2Lds = 20480;
 constexpr int kRes2Band = 4096;
@@ -81,6 +85,13 @@ struct FusedShapeOf {
 };
 template <int kBytes>
 struct FusedShapeOf<512, kBytes, 2> {
+    struct type {
+        static constexpr int kLds = kResLds;
+        static constexpr int kBand = FusedShape<512, kBytes>::kBand;
+    };
+};
+template <int kBytes>
+struct FusedShapeOf<512, kBytes, 3> {
     struct type {
         static constexpr int kLds = kResLds;
         static constexpr int kBand = FusedShape<512, kBytes>::kBand;
@@ -189,7 +200,8 @@ struct FusedLaunch {
     // kernels, nx a multiple of 4, no oversampling, stochastic rounding or
     // step limit), the record sets of all of a pass's back projections staged
     // once per launch: 0 off, 1 in the shape's LDS array, 2 in the 160 KiB one,
-    // 4 in the 80 KiB one with band_rows z rows per band
+    // 4 in the 80 KiB one with band_rows z rows per band; 3: off, with the
+    // band in the 160 KiB array and band_rows z rows per band
     int res;
     int band_rows;               // > 0: z rows per band instead of each member's own
     // non-null: nothing is launched; the number of workgroups of the kernel
@@ -466,12 +478,15 @@ __device__ __forceinline__ float bp_update_fast(const float4* pr, float xt, floa
 // roundings as on the LDS path, so the volumes are bit-identical.
 template <typename S, bool kVec, bool kCap, bool kFast, bool kMask, bool kOver, int kThreads, int RPT, bool kSR,
           int kRes>
-__global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <= 2 || (kOver && RPT == 4)) ? 4 : 1)
+__global__ void __launch_bounds__(kThreads, kRes == 2 || kRes == 3 ? 2
+                                            : (kRes == 4 || RPT <= 2 || (kOver && RPT == 4)) ? 4 : 1)
     fused_plane_kernel(const FusedVolume* __restrict__ vols, const FusedWork* __restrict__ work,
                        const FusedPass* const* __restrict__ passes, int band_rows)
 {
-    static_assert(kRes == 0 || (kVec && kFast && !kOver && !kSR), "register back projection: fast vector kernels");
-    static_assert((kRes != 2 && kRes != 4) || kThreads == 512, "resident record sets: 512-thread workgroups");
+    // kRes 1, 2, 4: the back projection in registers; 0 and 3: on the band in LDS
+    constexpr bool kReg = kRes == 1 || kRes == 2 || kRes == 4;
+    static_assert(!kReg || (kVec && kFast && !kOver && !kSR), "register back projection: fast vector kernels");
+    static_assert((kRes != 2 && kRes != 3 && kRes != 4) || kThreads == 512, "160 and 80 KiB arrays: 512 threads");
     using IO = GroupIo<S, kVec>;
     using Shape = typename FusedShapeOf<kThreads, (int)sizeof(typename S::T), kRes>::type;
     constexpr int G = IO::G;
@@ -508,7 +523,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
     const int nfp = pass->nfp, nbp = pass->nbp;
     // record sets staged at a time (set q: back projection q; the LDS path
     // restages set 0 per projection when a pass has several)
-    const int nsets = kRes ? max(nbp, 1) : 1;
+    const int nsets = kReg ? max(nbp, 1) : 1;
     const int set_len = np * rowsets * kRecStride * row_len;   // float4 per record set
     float2* pterm = reinterpret_cast<float2*>(recs + nsets * set_len);
     const int R = fv.rows_per_plane;
@@ -703,7 +718,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
         }
     }
     if (RPT > 2) prefetch(z0, z1, 0, kPre);
-    if constexpr (kRes != 0) {
+    if constexpr (kReg) {
         // every back projection's records, before the first band needs them
         if (nbp > 0) {
             for (int q = 0; q < nbp; ++q) stage_rows(q, q);
@@ -719,8 +734,8 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
     [[maybe_unused]] int rp = 0, rx0 = 0;
     [[maybe_unused]] const int rz0 = tid / gpr, rdz = kThreads / gpr;
     [[maybe_unused]] const float4* rrec = recs;   // the plane's records of set 0 from column -pad_l on
-    [[maybe_unused]] float xc[kRes ? 4 : 1];
-    if constexpr (kRes != 0) {
+    [[maybe_unused]] float xc[kReg ? 4 : 1];
+    if constexpr (kReg) {
         const int xw = (tid % gpr) * 4;
         if (np > 1) {
             rp = xw / nx;
@@ -740,7 +755,7 @@ __global__ void __launch_bounds__(kThreads, kRes == 2 ? 2 : (kRes == 4 || RPT <=
         constexpr int kSplit = sizeof(typename S::T) == 4 ? (kPre + 1) / 2 : kPre;
         int nb0 = 0, nb1 = 0;
         if (bi + 1 < nbands) band_range(bi + 1, nb0, nb1);
-        if constexpr (kRes != 0) {
+        if constexpr (kReg) {
             // 1. the prefetched band with the previous block's back projections
             //    applied in registers, then to LDS and HBM
             float v[kPre][4];
@@ -1198,6 +1213,16 @@ void launch_fused_rpt(const FusedLaunch& l, hipStream_t stream)
 template <typename S, bool kFast, bool kMask, bool kOver>
 void launch_fused_typed(bool vec, bool cap, const FusedLaunch& l, hipStream_t stream)
 {
+    if (l.res == 3) {
+        if constexpr (!kOver) {
+            if (vec && !cap && !l.sr && l.threads == 512 && l.band_rows > 0) {
+                launch_fused_shape<S, true, false, kFast, kMask, false, 512, false, 3>(l, stream);
+                return;
+            }
+        }
+        fail("fused plane kernel: the 160 KiB LDS back projection needs nx a multiple of 4, no oversampling, "
+             "stochastic rounding or step limit, and 512 threads");
+    }
     if (l.res != 0) {
         if constexpr (kFast && !kOver) {
             if (vec && !cap && !l.sr) {
