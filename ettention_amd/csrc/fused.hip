@@ -122,7 +122,7 @@ int res2_band(const MemberPlan& m, int sets, bool mask)
 // then streams only the first part of every z row of the volume (a contiguous
 // nx * ny * voxel-size slice per row), and with 4 MiB slices (1024 x 1024
 // f32) that part alone left HBM half idle: 1024 x 1024 x 256 f32
-// SART ran at 485 GUPS against 620 with this order.  Here the rounds' groups
+// SART ran at 485-489 GUPS against 605-620 with this order.  Here the rounds' groups
 // are interleaved: with r rounds of n groups each, dispatch i runs group
 // (i mod r) * n + i / r, so every round spans the whole slice in r contiguous
 // runs.  Every group still runs once and the groups are independent within a
