@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxel-type float|half|bfloat16]
                     [--workload metric|c2|c3|c4|c5|wbp] [--generic] [--lds-back-projection] [--mask]
-                    [--fp-samples N] [--bp-samples M] [--no-cpu-baseline]
+                    [--fp-samples N] [--bp-samples M] [--no-cpu-baseline] [--no-time-to-target]
 
 A step is one full iteration of the block-iterative loop over the synthetic
 tilt series (for SART: every projection's forward projection, residual and
@@ -12,7 +12,10 @@ the timed region starts.  Prints ONE JSON line on stdout; progress on stderr.
 --mask runs the metric and c2 workloads with a prior-knowledge mask (weight 1
 inside a y-axis cylinder holding the specimen, 0 outside); --fp-samples N
 (N x N sub-rays per pixel) and --bp-samples M (an M-point pattern per voxel)
-run them oversampled.
+run them oversampled.  The default metric line also carries
+`time_to_target`: bf16 SART with the learned jump (network parameters
+committed in ettention_amd/data) against plain FP32 SART on the c5 workload's
+held-out phantom (--no-time-to-target skips it).
 """
 from __future__ import annotations
 
@@ -87,6 +90,9 @@ def main():
                     help="projection steps the CPU baseline times (about 0.35 s each on 16 threads)")
     ap.add_argument("--train-phantoms", type=int, default=8, help="c5: training phantoms (512^3 each)")
     ap.add_argument("--train-steps", type=int, default=2000, help="c5: Adam steps on 128^3 crops")
+    ap.add_argument("--save-params", default="", help="c5: write the trained network's parameters (numpy .npy)")
+    ap.add_argument("--no-time-to-target", action="store_true",
+                    help="metric: skip the time-to-target measurement with the committed network parameters")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (traffic calibration for rocprof counters)")
     args = ap.parse_args()
@@ -241,7 +247,53 @@ def main():
     if args.fp_samples != 1 or args.bp_samples != 1:
         out["config"]["fp_samples"] = args.fp_samples
         out["config"]["bp_samples"] = args.bp_samples
+    if (args.workload == "metric" and args.voxel_type == "float" and plain and not args.mask and not args.generic
+            and not args.no_time_to_target):
+        del projs
+        torch.cuda.empty_cache()
+        out["time_to_target"] = committed_time_to_target(log)
     print(json.dumps(out), flush=True)
+
+
+NET_PARAMS = os.path.join(ROOT, "ettention_amd", "data", "correction_net_512.npy")
+
+
+def committed_time_to_target(log):
+    """The metric's second half: time to the target residual (the lowest
+    ||Ax - b|| plain FP32 SART reaches within 50 iterations) for bf16-storage
+    SART with the learned jump from iterate 3 against plain FP32 SART, on the
+    c5 workload's held-out 512^3 phantom (121 tilts, Poisson noise at 1e4
+    counts), with the network parameters committed in ettention_amd/data
+    (trained by `bench.py --workload c5 --save-params`).  The jump counts only
+    when accepted (its residual below plain SART's at equal wall time);
+    otherwise plain SART is the result and the speedup is 1."""
+    import torch
+    from ettention_amd import learned as L
+    if not os.path.exists(NET_PARAMS):
+        return {"skipped": "no committed network parameters"}
+    try:
+        wl = WORKLOADS["c5"]
+        n, N = wl["res"], wl["tilts"]
+        params = np.load(NET_PARAMS, allow_pickle=False)
+        t0 = time.time()
+        net = L.CorrectionNet((n, n, n), params)
+        case, ph = L.make_case((n, n, n), N, seed=5000)
+        del ph
+        res = L.time_to_target(case, net, 1.0, k_max=wl["k_max"], k0=wl["k0"], lam=wl["lam"], order=wl["order"],
+                               log=log)
+        del net, case
+        torch.cuda.empty_cache()
+        speedup = res["speedup"] if (res["accepted"] and res["speedup"]) else 1.0
+        log("time to target: %.2f s of measurement" % (time.time() - t0))
+        return {"speedup": round(speedup, 4), "unit": "x", "accepted": res["accepted"],
+                "target_norm": round(res["target_norm"], 2), "plain_best_iteration": res["plain_best_iteration"],
+                "plain_s": round(res["plain_s"], 4), "jump_s": round(res["jump_s"], 4),
+                "jump_norm": round(res["jump_norm"], 2), "plain_norm_at_jump_time":
+                round(res["plain_norm_at_jump_time"], 2), "learned_reached_target": res["reached"],
+                "learned_s": round(res["learned_s"], 4),
+                "workload": wl["name"], "params": os.path.relpath(NET_PARAMS, ROOT)}
+    except Exception as e:   # the metric line stands on its own
+        return {"error": "%s: %s" % (type(e).__name__, e)}
 
 
 def stream_copy_gbs(dev, floats=1 << 28, reps=5):
@@ -640,6 +692,9 @@ def run_learned(args):
     log("trained %d steps on %d^3 crops in %.1f s: mean loss of the first / last 100 steps %.4e -> %.4e"
         % (args.train_steps, crop, train_s, np.mean(losses[:100]), np.mean(losses[-100:])))
     params = net.parameters()
+    if args.save_params:
+        np.save(args.save_params, params.astype(np.float32), allow_pickle=False)
+        log("parameters written to %s" % args.save_params)
     del net, samples
     torch.cuda.empty_cache()
     full = L.CorrectionNet((n, n, n), params)
