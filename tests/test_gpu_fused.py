@@ -138,6 +138,45 @@ def test_rounds_of_plane_groups(synthetic):
     assert rel_l2(vols["fast"], vols["generic"]) <= 1e-5
 
 
+@pytest.mark.parametrize("kind", ["f32_mask", "exact", "bf16", "block4"])
+def test_wide_planes_take_the_large_lds_band(kind):
+    # 1536 rays per plane: four per thread, one 512-thread workgroup per CU,
+    # so the launches take the 160 KiB array with taller bands (the LDS back
+    # projection); against the generic kernels
+    nx, ny, nz = 1536, 4, 96
+    truth = S.ellipsoid_phantom((nz, ny, nx), seed=80)
+    vg = E.VolumeGeometry.make((nx, ny, nz))
+    base = E.ScannerGeometry.mrc_base(nx, ny)
+    tilts = S.tilt_series(60.0, 10.0)
+    truth_t = dev(truth)
+    projs = torch.zeros((len(tilts), ny, nx), dtype=torch.float32, device="cuda")
+    scratch = torch.zeros((ny, nx), dtype=torch.float32, device="cuda")
+    for i, t in enumerate(tilts):
+        A.forward_project(vg, truth_t, A.rotate(base, float(t)), projs[i], scratch)
+    kw = dict(volume=vg, base_geometry=base, tilt_angles=tilts, projections=projs, iterations=2, lam=0.3,
+              algorithm="sart", projection_iteration="random")
+    flags = 0
+    if kind == "f32_mask":
+        kw["mask"] = dev(seeded_mask((nx, ny, nz), 81))
+    if kind == "exact":   # (with a mask, exact records of 1536 columns leave no room for a band row)
+        flags = E.ETN_RECON_FLAG_EXACT_ARITHMETIC
+    if kind == "bf16":
+        kw["voxel_type"] = "bfloat16"
+    if kind == "block4":
+        kw.update(algorithm="blockiterative", block_size=4)
+    vols = {}
+    for name, f in (("fused", flags), ("generic", E.ETN_RECON_FLAG_GENERIC_KERNELS)):
+        r = A.BlockIterativeReconstructionOperator(flags=f, **kw)
+        assert r.uses_fused_path() == (name == "fused")
+        r.run()
+        r.synchronize()
+        vols[name] = r.get_reconstructed_volume(dtype=torch.float32).cpu().numpy()
+    if kind == "exact":
+        np.testing.assert_array_equal(vols["fused"], vols["generic"])
+    else:
+        assert rel_l2(vols["fused"], vols["generic"]) <= (1e-3 if kind == "bf16" else 1e-5)
+
+
 def test_ineligible_configurations_use_generic_kernels(synthetic):
     gt, tilts, projs = synthetic
     sub = slice(None, None, 20)
