@@ -8,7 +8,9 @@
 * A 1536 x 2 x 600 volume at tilts up to 60 degrees: nx + nz > 2049, but no
   ray can reach the limit, so the fused kernels run without it; exact fused
   arithmetic bit-identical to the generic kernels (which keep it), fast
-  within 1e-5, the generic kernels within 1e-5 of the oracle.
+  within 1e-5, the generic kernels within 1e-5 of the oracle.  The same at
+  1024 x 2 x 1024 with tilts up to 44 degrees, where a ray's bound (2015
+  voxels) sits just below the 2040 at which the limited kernels take over.
 * x-axis tilt (SingleAxisTiltRotator.cpp:62-78): rays leave the y planes, so
   the generic kernels run; SART, SIRT and weighted back projection against the
   oracle within 1e-5."""
@@ -115,6 +117,34 @@ def test_wide_volume_below_the_step_limit():
     spec = O.ReconSpec(ov, obase, tilts, projs, block_size=1, order="random", lam=0.3)
     ref = spec.run(np.zeros((nz, ny, nx), np.float32), 2)
     assert rel_l2(vols["generic"], ref) <= 1e-5
+    assert np.array_equal(vols["exact"], vols["generic"])
+    assert rel_l2(vols["fast"], vols["generic"]) <= 1e-5
+
+
+def test_step_limit_bound_near_the_limit():
+    # 1024 x 2 x 1024 at tilts up to 44 degrees: a ray can visit up to
+    # floor(1024 tan 44) + 1024 + 3 = 2015 voxels (the actual longest about
+    # 2013), just below the 2040 bound, so the fused kernels run without the
+    # limit; exact fused arithmetic must still equal the generic kernels
+    # (which keep it) bit for bit
+    shape = (1024, 2, 1024)
+    nx, ny, nz = shape
+    tilts = np.array([-44, -30, -5, 0, 20, 44], np.float32)
+    rng = np.random.default_rng(23)
+    truth = rng.uniform(0.0, 1.0, (nz, ny, nx)).astype(np.float32)
+    ov = O.Volume.make(shape)
+    obase = O.mrc_base_scanner(nx, ny)
+    projs = np.stack([O.forward_project(ov, truth, O.rotate(obase, float(t)))[0] for t in tilts])
+    print("near-limit SART", shape, flush=True)
+    kw = dict(volume=E.VolumeGeometry.make(shape), base_geometry=E.ScannerGeometry.mrc_base(nx, ny),
+              tilt_angles=tilts, projections=dev(projs), algorithm="sart", lam=0.3, projection_iteration="random")
+    vols = {}
+    for name, flags in (("generic", E.ETN_RECON_FLAG_GENERIC_KERNELS), ("exact", E.ETN_RECON_FLAG_EXACT_ARITHMETIC),
+                        ("fast", 0)):
+        r = A.BlockIterativeReconstructionOperator(flags=flags, **kw)
+        assert r.uses_fused_path() == (name != "generic"), name
+        r.run_iterations(2)
+        vols[name] = r.get_reconstructed_volume().cpu().numpy()
     assert np.array_equal(vols["exact"], vols["generic"])
     assert rel_l2(vols["fast"], vols["generic"]) <= 1e-5
 
