@@ -834,7 +834,20 @@ def cpu_baseline(wl, tilts, projs, nsample, mask=None, fp_samples=1, bp_samples=
             "sample": "%d SART projection steps (FP+residual+BP) on the full %dx%dx%d volume, %d^2 detector%s%s"
                       % (nsample, nx, ny, nz, nu, ", masked" if mask is not None else "",
                          ", FP %dx%d sub-rays, BP %d samples" % (fp_samples, fp_samples, bp_samples)
-                         if fp_samples != 1 or bp_samples != 1 else "")}
+                         if fp_samples != 1 or bp_samples != 1 else ""),
+            "cpu_model": host_cpu_model(), "build": "gcc -O2 -ffp-contract=off -fopenmp (FP32 as written)"}
+
+
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None where it is not readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":
