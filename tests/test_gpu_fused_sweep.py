@@ -2,7 +2,7 @@
 shape (nx a multiple of 4 or not, odd sizes), detector pitch, algorithm and
 block size, projection order, long-object compensation, z-slabs, mask,
 forward and back-projection oversampling, storage type, workgroup shape and
-tile.  With the default arithmetic and f32 storage every combination must
+tile; and wide planes of 512 to 2048 columns.  With the default arithmetic and f32 storage every combination must
 stay within the FP32 parity bound (relative L2 1e-5) of the same
 reconstruction on the generic kernels, whichever path the options allow.
 With bf16 storage the two paths round differently and their volumes drift
@@ -50,9 +50,33 @@ def config(seed):
     return c
 
 
-@pytest.mark.parametrize("seed", range(100, 140))
+def wide_config(seed):
+    """Wide planes (512 to 2048 columns): four or more rays per thread, so the
+    launches on the LDS path take the 160 KiB array, and 2048 columns put
+    nx + nz past 2049 with rays that cannot reach the step limit."""
+    rng = np.random.default_rng(seed)
+    nx = int(rng.choice([512, 1024, 1536, 2048]))
+    c = dict(shape=(nx, int(rng.choice([2, 4, 6])), int(rng.choice([24, 48, 96]))),
+             algorithm=str(rng.choice(["sart", "sirt", "blockiterative"])), lam=float(rng.choice([0.2, 0.5])),
+             projection_iteration=str(rng.choice(["identity", "random", "maxangle"])),
+             tilt_step=float(rng.choice([6.0, 10.0])), pitch=1.0 if nx > 1024 else float(rng.choice([1.0, 0.5])))
+    if c["algorithm"] == "blockiterative":
+        c["block_size"] = int(rng.integers(2, 9))
+    if rng.random() < 0.3:
+        c["mask"] = True
+    if rng.random() < 0.2:
+        c["use_long_object_compensation"] = True
+    if rng.random() < 0.2:
+        c["voxel_type"] = "bfloat16"
+    c["threads"] = 512
+    c["seed"] = seed
+    return c
+
+
+@pytest.mark.parametrize("seed", list(range(100, 140)) + ["wide%d" % k for k in range(12)])
 def test_fused_option_sweep_matches_generic(seed):
-    c = config(seed)
+    c = wide_config(200 + int(seed[4:])) if isinstance(seed, str) else config(seed)
+    seed = c["seed"]
     print("sweep case", c, flush=True)
     nx, ny, nz = c["shape"]
     vg = E.VolumeGeometry.make((nx, ny, nz), subvolumes=c.get("subvolumes", 1))
