@@ -104,6 +104,21 @@ def test_batch_graph_replays_equal_eager(members):
         np.testing.assert_array_equal(a, c)
 
 
+def test_batch_with_a_member_in_rounds():
+    # a 512 x 1024 x 8 member has 1024 one-plane groups, more than the device
+    # holds at once: its groups are dispatched interleaved, the small member's
+    # in order; with exact arithmetic each equals its generic run alone
+    sel = [member((512, 1024, 8), 13, 4010), member((64, 48, 40), 21, 4011)]
+    b = A.BatchReconstruction(sel, flags=EXACT)
+    assert all(b.uses_fused_path(i) for i in range(len(b)))
+    b.run_iterations(2)
+    b.synchronize()
+    for i, m in enumerate(sel):
+        r, ref = alone(m, 2)
+        np.testing.assert_array_equal(b.get_reconstructed_volume(i, dtype=torch.float32).cpu().numpy(), ref,
+                                      err_msg="member %d" % i)
+
+
 def test_batch_all_generic_flag(members):
     sel = members[2:4]
     b = A.BatchReconstruction(sel, flags=E.ETN_RECON_FLAG_GENERIC_KERNELS)
