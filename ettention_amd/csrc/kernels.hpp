@@ -77,6 +77,10 @@ constexpr uint32_t kMaxRampRow = 8192;    // padded row length held in LDS
 constexpr uint32_t kMaxRampTaps = 512;
 void launch_ramp_filter(const RampFilterParams& p, const float* taps, const float* src, float* dst,
                         hipStream_t stream);
+// `count` images in one launch: image i from src + i * src_stride to
+// dst + i * dst_stride (floats), each filtered as launch_ramp_filter does.
+void launch_ramp_filter_batch(const RampFilterParams& p, const float* taps, const float* src, size_t src_stride,
+                              float* dst, size_t dst_stride, uint32_t count, hipStream_t stream);
 void launch_zero(float* dst, size_t count, hipStream_t stream);
 
 }  // namespace dev

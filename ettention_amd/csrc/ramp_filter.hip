@@ -26,13 +26,16 @@ __device__ __forceinline__ int mirror(int x, int begin, int end, int size)
     return min(max(m, 0), size - 1);
 }
 
+// One workgroup per detector row (blockIdx.x) of image blockIdx.y.
 __global__ void __launch_bounds__(kRampThreads) ramp_filter_kernel(RampFilterParams p, const float* __restrict__ taps,
-                                                                   const float* __restrict__ src,
-                                                                   float* __restrict__ dst)
+                                                                   const float* __restrict__ src, size_t src_stride,
+                                                                   float* __restrict__ dst, size_t dst_stride)
 {
     __shared__ float row[kMaxRampRow];
     __shared__ float tap[kMaxRampTaps];
     const uint32_t v = blockIdx.x;
+    src += (size_t)blockIdx.y * src_stride;
+    dst += (size_t)blockIdx.y * dst_stride;
     // the shifted filter sits on padded row -1, so output row py reads padded row py + 1
     const int py = (int)((v + (uint32_t)p.crop_y + 1u) % p.uy);
     const int sy = mirror(py, p.pad_y, p.end_y, (int)p.nv);
@@ -42,26 +45,51 @@ __global__ void __launch_bounds__(kRampThreads) ramp_filter_kernel(RampFilterPar
     __syncthreads();
     const int half = (int)(p.taps / 2);
     const int ux = (int)p.ux;
-    for (uint32_t u = threadIdx.x; u < p.nu; u += kRampThreads) {
-        const int px = (int)u + p.crop_x;
-        float acc = 0.0f;
+    // |t - half| <= taps / 2 <= ux / 2 and the cropped columns lie in the
+    // padded row: one wrap suffices
+    auto at = [&](int x) { return row[x < 0 ? x + ux : (x >= ux ? x - ux : x)]; };
+    // four consecutive outputs per thread: tap t reads padded column
+    // px - (t - half) for each, so the four columns slide down by one per tap
+    // and one new column is read per tap (the sums run over t in order, as one
+    // output per thread would)
+    const int groups = (int)((p.nu + 3) / 4);
+    for (int g = (int)threadIdx.x; g < groups; g += kRampThreads) {
+        const int px0 = 4 * g + p.crop_x;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] = at(px0 + j + half);
         for (int t = 0; t < (int)p.taps; ++t) {
-            int x = px - (t - half);   // |t - half| <= taps / 2 <= ux / 2: one wrap suffices
-            x = x < 0 ? x + ux : (x >= ux ? x - ux : x);
-            acc = fmaf(tap[t], row[x], acc);
+            const float c = tap[t];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j] = fmaf(c, w[j], acc[j]);
+            w[3] = w[2];
+            w[2] = w[1];
+            w[1] = w[0];
+            w[0] = at(px0 - (t + 1) + half);
         }
-        dst[(size_t)v * p.nu + u] = acc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (4 * g + j < (int)p.nu) dst[(size_t)v * p.nu + 4 * g + j] = acc[j];
     }
 }
 
 }  // namespace
 
-void launch_ramp_filter(const RampFilterParams& p, const float* taps, const float* src, float* dst, hipStream_t stream)
+void launch_ramp_filter_batch(const RampFilterParams& p, const float* taps, const float* src, size_t src_stride,
+                              float* dst, size_t dst_stride, uint32_t count, hipStream_t stream)
 {
     if (p.ux > kMaxRampRow || p.taps > kMaxRampTaps || p.taps > p.ux || p.nu > p.ux || p.nv > p.uy || p.nv == 0)
         fail("ramp filter: padded size out of range");
-    ramp_filter_kernel<<<p.nv, kRampThreads, 0, stream>>>(p, taps, src, dst);
+    if (count == 0) return;
+    if (count > 65535) fail("ramp filter: at most 65535 images per launch");
+    ramp_filter_kernel<<<dim3(p.nv, count), kRampThreads, 0, stream>>>(p, taps, src, src_stride, dst, dst_stride);
     ETN_HIP(hipGetLastError());
+}
+
+void launch_ramp_filter(const RampFilterParams& p, const float* taps, const float* src, float* dst, hipStream_t stream)
+{
+    launch_ramp_filter_batch(p, taps, src, 0, dst, 0, 1, stream);
 }
 
 }  // namespace dev
