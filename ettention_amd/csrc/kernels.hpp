@@ -47,6 +47,10 @@ void launch_compare(uint32_t n, const void* real, etn_voxel_type real_type, cons
                     hipStream_t stream);
 void launch_long_object_compensate(uint32_t n, float* projection, float* length, float c, hipStream_t stream);
 void launch_ray_length(const float bmin[3], const float bmax[3], const RayParams& r, float* out, hipStream_t stream);
+// `count` images of nu x nv rays in one launch: image i with the geometry
+// params[i] (device memory, every nu and nv the same) at out + i * stride
+void launch_ray_length_batch(const float bmin[3], const float bmax[3], const RayParams* params, uint32_t count,
+                             uint32_t nu, uint32_t nv, float* out, size_t stride, hipStream_t stream);
 // partials must hold ssd_partial_count() doubles
 constexpr uint32_t kSsdBlocks = 512;
 void launch_sum_squared_difference(const float* a, const float* b, size_t count, double* partials, double* out,

@@ -97,14 +97,10 @@ __global__ void loc_kernel(uint32_t n, float* proj, float* len, float c)
     len[i] = c;
 }
 
-// ComputeRayLength.cl:execute
-__global__ void ray_length_kernel(RayParams g, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
-                                  float* out)
+// ComputeRayLength.cl:execute for pixel (x, y)
+__device__ __forceinline__ float ray_length_at(const RayParams& g, const float bmin[3], const float bmax[3],
+                                               uint32_t x, uint32_t y)
 {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= g.nu || y >= g.nv) return;
-    const float bmin[3] = {bx0, by0, bz0}, bmax[3] = {bx1, by1, bz1};
     float src[3];
     const float fx = ((float)x) + 0.5f, fy = ((float)y) + 0.5f;
     float tmin[3], tmax[3];
@@ -119,7 +115,29 @@ __global__ void ray_length_kernel(RayParams g, float bx0, float by0, float bz0, 
     }
     const float t_in = fmaxf(fmaxf(tmin[0], tmin[1]), tmin[2]);
     const float t_out = fminf(fminf(tmax[0], tmax[1]), tmax[2]);
-    out[(size_t)y * g.nu + x] = fmaxf(t_out - t_in, 0.0f);
+    return fmaxf(t_out - t_in, 0.0f);
+}
+
+__global__ void ray_length_kernel(RayParams g, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+                                  float* out)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= g.nu || y >= g.nv) return;
+    const float bmin[3] = {bx0, by0, bz0}, bmax[3] = {bx1, by1, bz1};
+    out[(size_t)y * g.nu + x] = ray_length_at(g, bmin, bmax, x, y);
+}
+
+// Image blockIdx.z of a batch: its geometry from gs, its lengths at out + z * stride.
+__global__ void ray_length_batch_kernel(const RayParams* __restrict__ gs, float bx0, float by0, float bz0, float bx1,
+                                        float by1, float bz1, float* out, size_t stride)
+{
+    const RayParams g = gs[blockIdx.z];
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= g.nu || y >= g.nv) return;
+    const float bmin[3] = {bx0, by0, bz0}, bmax[3] = {bx1, by1, bz1};
+    out[(size_t)blockIdx.z * stride + (size_t)y * g.nu + x] = ray_length_at(g, bmin, bmax, x, y);
 }
 
 // FP64 sum of squared differences: per-block partials, then one block sums
@@ -269,6 +287,18 @@ void launch_ray_length(const float bmin[3], const float bmax[3], const RayParams
     const dim3 block(64, 4);
     const dim3 grid((r.nu + 63) / 64, (r.nv + 3) / 4);
     ray_length_kernel<<<grid, block, 0, stream>>>(r, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], out);
+    check_launch("ray length launch");
+}
+
+void launch_ray_length_batch(const float bmin[3], const float bmax[3], const RayParams* params, uint32_t count,
+                             uint32_t nu, uint32_t nv, float* out, size_t stride, hipStream_t stream)
+{
+    if (count == 0) return;
+    if (count > 65535) fail("ray length: at most 65535 images per launch");
+    const dim3 block(64, 4);
+    const dim3 grid((nu + 63) / 64, (nv + 3) / 4, count);
+    ray_length_batch_kernel<<<grid, block, 0, stream>>>(params, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2],
+                                                        out, stride);
     check_launch("ray length launch");
 }
 
