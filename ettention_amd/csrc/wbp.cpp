@@ -96,6 +96,14 @@ WeightedBackProjection::WeightedBackProjection(const etn_reconstruction_desc& d,
     if ((d.flags & (ETN_RECON_FLAG_GENERIC_KERNELS | ETN_RECON_FLAG_EXACT_ARITHMETIC)) == 0)
         fused_ = FusedBackProjector::create(volume_.get(), geometries_, nu, nv, d.bp_samples ? d.bp_samples : 1,
                                             (d.flags & ETN_RECON_FLAG_LDS_BACK_PROJECTION) != 0, stream_);
+    if (fused_) {
+        std::vector<dev::RayParams> geo;
+        for (const ParallelScannerGeometry& g : geometries_) geo.push_back(make_ray_params(g, 1));
+        rays_.allocate(sizeof(dev::RayParams) * std::max<size_t>(geo.size(), 1));
+        ETN_HIP(hipMemcpyAsync(rays_.get(), geo.data(), sizeof(dev::RayParams) * geo.size(), hipMemcpyHostToDevice,
+                               stream_));
+        ETN_HIP(hipStreamSynchronize(stream_));
+    }
 }
 
 WeightedBackProjection::~WeightedBackProjection()
@@ -119,7 +127,6 @@ void WeightedBackProjection::runNiterations(uint32_t n)
         // back projections in stack order, eight per volume sweep
         constexpr uint32_t kChunk = 64;
         DeviceBuffer chunk(sizeof(float) * 2 * pixels_ * std::min(n, kChunk));
-        DeviceBuffer rays(sizeof(dev::RayParams) * std::min(n, kChunk));
         const VolumeProperties& vp = setup_->getVolume();
         float bmin[3], bmax[3];
         vp.bbox_min().store(bmin);
@@ -133,11 +140,8 @@ void WeightedBackProjection::runNiterations(uint32_t n)
             // each, most of it launch and ramp-up)
             dev::launch_ramp_filter_batch(params_, taps_.as<float>(), projections_ + (size_t)i0 * pixels_, pixels_,
                                           chunk.as<float>(), 2 * pixels_, i1 - i0, stream_);
-            std::vector<dev::RayParams> geo;
-            for (uint32_t i = i0; i < i1; ++i) geo.push_back(make_ray_params(geometries_[i], 1));
-            ETN_HIP(hipMemcpyAsync(rays.get(), geo.data(), sizeof(dev::RayParams) * geo.size(), hipMemcpyHostToDevice,
-                                   stream_));
-            dev::launch_ray_length_batch(bmin, bmax, rays.as<dev::RayParams>(), i1 - i0, geo[0].nu, geo[0].nv,
+            dev::launch_ray_length_batch(bmin, bmax, rays_.as<dev::RayParams>() + i0, i1 - i0,
+                                         geometries_[i0].nu(), geometries_[i0].nv(),
                                          chunk.as<float>() + pixels_, 2 * pixels_, stream_);
             for (uint32_t i = i0; i < i1; ++i) {
                 float* f = chunk.as<float>() + 2 * pixels_ * (i - i0);
@@ -145,8 +149,8 @@ void WeightedBackProjection::runNiterations(uint32_t n)
                 res.push_back(f);
                 len.push_back(f + pixels_);
             }
-            // backProject synchronises the stream before it returns: geo and
-            // the buffers stay valid while the copies and launches read them
+            // backProject synchronises the stream before it returns, so the
+            // next chunk may reuse the buffer
             fused_->backProject(idx, res, len, lambda_ / (float)count_, stream_, launch_ms_);
         }
         return;

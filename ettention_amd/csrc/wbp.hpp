@@ -66,6 +66,7 @@ private:
     DeviceBuffer taps_;
     dev::RampFilterParams params_;
     std::unique_ptr<FusedBackProjector> fused_;
+    DeviceBuffer rays_;   // fused path: every projection's ray geometry, for the batched ray lengths
     std::vector<float>* launch_ms_ = nullptr;    // set while timeKernels runs
 };
 
