@@ -100,7 +100,10 @@ int max_band(const MemberPlan& m, int planes) { return max_band(m, planes, m.thr
 
 // The 80 KiB register back projection (dev::FusedLaunch::res == 4): LDS
 // floats of member m's band of `band` rows with `sets` record sets staged, and
-// the tallest band (at most the member's own) that fits; 0 when none does.
+// the tallest band (at most the member's own) that fits; 0 when none does or
+// when it holds less than half the 80 KiB path's band cap (with the records
+// leaving only a row or two, the 160 KiB array's one workgroup per CU with its
+// full bands is the faster choice).
 size_t res2_lds_bytes(const MemberPlan& m, int band, int sets, bool mask)
 {
     const int width = m.tile.planes * m.nx;
@@ -114,7 +117,8 @@ int res2_band(const MemberPlan& m, int sets, bool mask)
     const int rec = m.tile.planes * m.rowsets * sets * (m.row_floats + 2);
     const int room = dev::kRes2Lds - (mask ? dev::kRes2Band / 2 : 0) - rec - 3;
     if (room < width) return 0;
-    return std::min(m.tile.band, std::min(cap / width, room / width));
+    const int band = std::min(m.tile.band, std::min(cap / width, room / width));
+    return 2 * band * width >= cap || band == m.tile.band ? band : 0;
 }
 
 // Dispatch order of a launch's workgroups.  The plane groups of a launch that
@@ -1112,10 +1116,17 @@ void FusedBackProjectorImpl::backProject(const std::vector<uint32_t>& indices,
     if (residual.size() != indices.size() || length.size() != indices.size())
         fail("fused back projector: one residual and one length image per projection");
     const VolumeProperties vp = whole_volume(volume_->properties());
+    // passes of at most kMaxFusedBp projections, as even as the count allows
+    // (121 = 15 x 8 + 1 would spend a whole volume sweep on the last one)
+    const size_t npass = (indices.size() + dev::kMaxFusedBp - 1) / dev::kMaxFusedBp;
     std::vector<dev::FusedPass> passes;
+    size_t pass_end = 0;
     for (size_t k = 0; k < indices.size(); ++k) {
         if (indices[k] >= geometries_.size()) fail("fused back projector: projection index out of range");
-        if (k % dev::kMaxFusedBp == 0) {
+        if (k == pass_end) {
+            // pass p holds floor(n / npass) projections, one more for the first n mod npass
+            const size_t p = passes.size();
+            pass_end += indices.size() / npass + (p < indices.size() % npass ? 1 : 0);
             passes.push_back(dev::FusedPass{});
             passes.back().zdir = 1;
         }
